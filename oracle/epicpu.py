@@ -1,0 +1,239 @@
+"""ctypes binding of the CPU oracle (oracle/epicpu.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker. The product package
+(epirust_amd) never imports this module.
+
+`config_from_json` restates the reference's serde schema for the standalone
+Config (common/src/config/mod.rs:44-58): unknown keys are ignored, `disease_overrides`
+and `output_file` are accepted and unused, `starting_infections` defaults to
+`exposed: 1` (common/src/config/starting_infections.rs:65-69), and the project's
+extra key `seed` selects the Philox key.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libepicpu.so")
+MAX_VACC = 4096
+
+
+class EpicpuConfig(ctypes.Structure):
+    _fields_ = [
+        ("number_of_agents", ctypes.c_uint32),
+        ("public_transport_percentage", ctypes.c_double),
+        ("working_percentage", ctypes.c_double),
+        ("regular_transmission_start_day", ctypes.c_uint32),
+        ("high_transmission_start_day", ctypes.c_uint32),
+        ("last_day", ctypes.c_uint32),
+        ("asymptomatic_last_day", ctypes.c_uint32),
+        ("mild_infected_last_day", ctypes.c_uint32),
+        ("regular_transmission_rate", ctypes.c_double),
+        ("high_transmission_rate", ctypes.c_double),
+        ("death_rate", ctypes.c_double),
+        ("percentage_asymptomatic_population", ctypes.c_double),
+        ("percentage_severe_infected_population", ctypes.c_double),
+        ("exposed_duration", ctypes.c_uint32),
+        ("pre_symptomatic_duration", ctypes.c_uint32),
+        ("grid_size", ctypes.c_uint32),
+        ("hospital_beds_percentage", ctypes.c_double),
+        ("hours", ctypes.c_uint32),
+        ("si_infected_mild_asymptomatic", ctypes.c_uint32),
+        ("si_infected_mild_symptomatic", ctypes.c_uint32),
+        ("si_infected_severe", ctypes.c_uint32),
+        ("si_exposed", ctypes.c_uint32),
+        ("has_lockdown", ctypes.c_uint32),
+        ("lockdown_at_number_of_infections", ctypes.c_uint32),
+        ("lockdown_essential_workers_population", ctypes.c_double),
+        ("has_build_new_hospital", ctypes.c_uint32),
+        ("spread_rate_threshold", ctypes.c_uint32),
+        ("n_vaccinate", ctypes.c_uint32),
+        ("vaccinate_at_hour", ctypes.c_uint32 * MAX_VACC),
+        ("vaccinate_percent", ctypes.c_double * MAX_VACC),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+_P = ctypes.POINTER
+AGENT_FIELDS = [
+    ("x", np.int32, 1),
+    ("y", np.int32, 1),
+    ("state", np.uint8, 1),
+    ("severity", np.uint8, 1),
+    ("infection_day", np.uint32, 1),
+    ("at_hour", np.uint32, 1),
+    ("flags", np.uint8, 1),
+    ("work_status", np.uint8, 1),
+    ("work_start_at", np.uint32, 1),
+    ("immunity", np.int8, 1),
+    ("current_area", np.int32, 4),
+]
+
+
+class EpicpuAgents(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name, _, _ in AGENT_FIELDS]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library missing: build it with `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.epicpu_create.restype = ctypes.c_void_p
+        L.epicpu_create.argtypes = [_P(EpicpuConfig), ctypes.c_char_p, ctypes.c_size_t]
+        L.epicpu_destroy.argtypes = [ctypes.c_void_p]
+        L.epicpu_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_uint32), _P(ctypes.c_uint32),
+                                  _P(ctypes.c_int)]
+        L.epicpu_read_agents.argtypes = [ctypes.c_void_p, _P(EpicpuAgents)]
+        L.epicpu_counts.argtypes = [ctypes.c_void_p, _P(ctypes.c_uint32)]
+        L.epicpu_events.restype = ctypes.c_uint32
+        L.epicpu_events.argtypes = [ctypes.c_void_p, _P(ctypes.c_uint32), _P(ctypes.c_uint32), ctypes.c_uint32]
+        L.epicpu_geometry.argtypes = [ctypes.c_void_p, _P(ctypes.c_int32), _P(ctypes.c_uint32)]
+        L.epicpu_philox4x32_10.argtypes = [_P(ctypes.c_uint32), _P(ctypes.c_uint32), _P(ctypes.c_uint32)]
+        L.epicpu_define_geography.argtypes = [ctypes.c_uint32, _P(ctypes.c_int32)]
+        L.epicpu_area_factory.restype = ctypes.c_uint32
+        L.epicpu_area_factory.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_uint32, _P(ctypes.c_int32), ctypes.c_uint32]
+        L.epicpu_resize_hospital.argtypes = [_P(ctypes.c_int32), ctypes.c_int32, ctypes.c_double, ctypes.c_double]
+        L.epicpu_increase_hospital_size.argtypes = [_P(ctypes.c_int32), ctypes.c_uint32]
+        L.epicpu_transmission_rate.restype = ctypes.c_double
+        L.epicpu_transmission_rate.argtypes = [_P(EpicpuConfig), ctypes.c_uint32]
+        L.epicpu_is_to_be_hospitalized.argtypes = [_P(EpicpuConfig), ctypes.c_uint32]
+        L.epicpu_neighbors_in_area.restype = ctypes.c_uint32
+        L.epicpu_neighbors_in_area.argtypes = [ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_int32), _P(ctypes.c_int32)]
+        L.epicpu_area_iter.restype = ctypes.c_uint32
+        L.epicpu_area_iter.argtypes = [_P(ctypes.c_int32), _P(ctypes.c_int32), ctypes.c_uint32]
+        L.epicpu_bernoulli_threshold.restype = ctypes.c_uint64
+        L.epicpu_bernoulli_threshold.argtypes = [ctypes.c_double]
+        L.epicpu_lockdown_should_apply.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.epicpu_lockdown_should_unlock.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
+        L.epicpu_hospital_counts_updated.restype = ctypes.c_uint32
+        L.epicpu_hospital_counts_updated.argtypes = [ctypes.c_uint32] * 3
+        L.epicpu_hospital_should_apply.argtypes = [ctypes.c_int] + [ctypes.c_uint32] * 4
+        L.epicpu_first_vacant.argtypes = [_P(ctypes.c_int32), _P(ctypes.c_int32), ctypes.c_uint32, _P(ctypes.c_int32)]
+        _lib = L
+    return _lib
+
+
+def _arr(ctype, values):
+    return (ctype * len(values))(*values)
+
+
+def config_from_json(cfg, seed=None):
+    """Build an EpicpuConfig from a reference-schema config (dict or JSON text)."""
+    if isinstance(cfg, (str, bytes)):
+        cfg = json.loads(cfg)
+    c = EpicpuConfig()
+    pop = cfg["population"]
+    if "Auto" not in pop:
+        raise ValueError("only Population::Auto is supported by the oracle")
+    auto = pop["Auto"]
+    c.number_of_agents = auto["number_of_agents"]
+    c.public_transport_percentage = auto["public_transport_percentage"]
+    c.working_percentage = auto["working_percentage"]
+    d = cfg["disease"]
+    for k in ("regular_transmission_start_day", "high_transmission_start_day", "last_day", "asymptomatic_last_day",
+              "mild_infected_last_day", "regular_transmission_rate", "high_transmission_rate", "death_rate",
+              "percentage_asymptomatic_population", "percentage_severe_infected_population", "exposed_duration",
+              "pre_symptomatic_duration"):
+        setattr(c, k, d[k])
+    geo = cfg["geography_parameters"]
+    c.grid_size = geo["grid_size"]
+    c.hospital_beds_percentage = geo["hospital_beds_percentage"]
+    c.hours = cfg["hours"]
+    si = cfg.get("starting_infections", {"infected_mild_asymptomatic": 0, "infected_mild_symptomatic": 0,
+                                          "infected_severe": 0, "exposed": 1})
+    c.si_infected_mild_asymptomatic = si["infected_mild_asymptomatic"]
+    c.si_infected_mild_symptomatic = si["infected_mild_symptomatic"]
+    c.si_infected_severe = si["infected_severe"]
+    c.si_exposed = si["exposed"]
+    nv = 0
+    for iv in cfg.get("interventions", []):
+        (kind, body), = iv.items()
+        if kind == "Lockdown" and not c.has_lockdown:
+            c.has_lockdown = 1
+            c.lockdown_at_number_of_infections = body["at_number_of_infections"]
+            c.lockdown_essential_workers_population = body["essential_workers_population"]
+        elif kind == "BuildNewHospital" and not c.has_build_new_hospital:
+            c.has_build_new_hospital = 1
+            c.spread_rate_threshold = body["spread_rate_threshold"]
+        elif kind == "Vaccinate":
+            c.vaccinate_at_hour[nv] = body["at_hour"]
+            c.vaccinate_percent[nv] = body["percent"]
+            nv += 1
+    c.n_vaccinate = nv
+    c.seed = int(seed if seed is not None else cfg.get("seed", 0)) & 0xFFFFFFFFFFFFFFFF
+    return c
+
+
+def alloc_agents(n):
+    arrays = {name: np.zeros(n * width, dtype=dt) for name, dt, width in AGENT_FIELDS}
+    s = EpicpuAgents(**{name: a.ctypes.data for name, a in arrays.items()})
+    return arrays, s
+
+
+class OracleSim:
+    """One deterministic-mode simulation on the CPU oracle."""
+
+    def __init__(self, cfg, seed=None):
+        self.cfg = config_from_json(cfg, seed)
+        err = ctypes.create_string_buffer(512)
+        self.h = lib().epicpu_create(ctypes.byref(self.cfg), err, 512)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        self.n = self.cfg.number_of_agents
+
+    def close(self):
+        if self.h:
+            lib().epicpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n_hours):
+        rows = np.zeros((max(n_hours, 1), 7), dtype=np.uint32)
+        nw = ctypes.c_uint32(0)
+        st = ctypes.c_int(0)
+        rc = lib().epicpu_step(self.h, n_hours, rows.ctypes.data_as(_P(ctypes.c_uint32)), ctypes.byref(nw),
+                               ctypes.byref(st))
+        if rc != 0:
+            raise RuntimeError("epicpu_step failed: %d" % rc)
+        return rows[: nw.value], bool(st.value)
+
+    def agents(self):
+        arrays, s = alloc_agents(self.n)
+        lib().epicpu_read_agents(self.h, ctypes.byref(s))
+        arrays["current_area"] = arrays["current_area"].reshape(-1, 4)
+        return arrays
+
+    def counts(self):
+        out = (ctypes.c_uint32 * 7)()
+        lib().epicpu_counts(self.h, out)
+        return list(out)
+
+    def events(self):
+        cap = 65536
+        hours = (ctypes.c_uint32 * cap)()
+        kinds = (ctypes.c_uint32 * cap)()
+        n = lib().epicpu_events(self.h, hours, kinds, cap)
+        return [(hours[i], kinds[i]) for i in range(min(n, cap))]
+
+    def geometry(self):
+        r = (ctypes.c_int32 * 20)()
+        c = (ctypes.c_uint32 * 3)()
+        lib().epicpu_geometry(self.h, r, c)
+        rects = [tuple(r[4 * i:4 * i + 4]) for i in range(5)]
+        return rects, list(c)
+
+
+def philox(ctr, key):
+    out = (ctypes.c_uint32 * 4)()
+    lib().epicpu_philox4x32_10(_arr(ctypes.c_uint32, ctr), _arr(ctypes.c_uint32, key), out)
+    return list(out)
