@@ -1,0 +1,809 @@
+// Host runtime of the C ABI (include/epirust_gpu.h): one handle = a batch of one or more
+// independent simulations stepped together on one HIP stream.
+#include <hip/hip_runtime_api.h>
+
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "epirust_gpu.h"
+#include "config.hpp"
+#include "kernels.hpp"
+#include "model.hpp"
+#include "population.hpp"
+
+using namespace epi;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) return fail(EPI_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+bool valid_p(double p) { return p >= 0.0 && p <= 1.0; }
+
+// is_to_be_hospitalized per transmission window (common/src/disease/mod.rs:97-99)
+uint32_t hospital_window_bits(const Config& c) {
+  uint32_t bits = 0;
+  if (0.0 >= c.high_transmission_rate) bits |= 1u;
+  if (c.regular_transmission_rate >= c.high_transmission_rate) bits |= 2u;
+  if (c.high_transmission_rate >= c.high_transmission_rate) bits |= 4u;
+  return bits;
+}
+
+uint8_t host_grid_byte(const Config& c, uint32_t w) {
+  if (st_state(w) != ST_I || (w & F_HOSP)) return 1;
+  const uint32_t d = static_cast<uint32_t>(static_cast<int32_t>(st_day(w)) + st_imm(w));
+  if (c.regular_transmission_start_day < d && d <= c.high_transmission_start_day) return 2;
+  if (c.high_transmission_start_day < d && d <= c.last_day) return 3;
+  return 1;
+}
+
+Rect building_rect(int32_t sx, int32_t sy, int32_t nx, int32_t size, uint32_t k) {
+  int32_t x0 = sx + static_cast<int32_t>(k % static_cast<uint32_t>(nx)) * size;
+  int32_t y0 = sy + static_cast<int32_t>(k / static_cast<uint32_t>(nx)) * size;
+  return Rect{x0, y0, x0 + size - 1, y0 + size - 1};
+}
+
+bool rect_inside(const Rect& r, uint32_t stride, uint32_t height) {
+  return r.x0 >= 0 && r.y0 >= 0 && r.x1 < static_cast<int32_t>(stride) && r.y1 < static_cast<int32_t>(height) &&
+         r.x0 <= r.x1 && r.y0 <= r.y1;
+}
+
+struct DeviceBuffers {
+  std::vector<void*> owned;
+  template <class T>
+  hipError_t alloc(T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) owned.push_back(q);
+    *p = static_cast<T*>(q);
+    return e;
+  }
+  ~DeviceBuffers() {
+    for (void* p : owned) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+struct epi_sim {
+  std::vector<Config> cfg;
+  std::vector<uint64_t> seed;
+  std::vector<Geometry> geo;
+  std::vector<SimDesc> desc;
+  std::vector<uint32_t> counts0;  // 7 per sim
+  uint32_t n_sims = 0;
+  uint64_t n_agents = 0, n_cells = 0, n_rows = 0, n_events = 0;
+  uint32_t max_hour = 0;  // largest hour any simulation can reach
+  bool any_empty = false;
+  uint32_t flags = 0;
+  LaunchShape shape;
+  std::vector<BlockRef> agent_blocks, hosp_blocks;
+  std::vector<uint32_t> vacc_hour;
+  std::vector<unsigned long long> vacc_thr;
+  DeviceBuffers mem;
+  DevView v{};
+  BlockRef* d_agent_blocks = nullptr;
+  SimDesc* d_desc = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t day_graph = nullptr;
+  hipEvent_t poll_event[2] = {nullptr, nullptr};
+  uint32_t* pinned_active = nullptr;  // 2 x n_sims
+  uint32_t host_hour = 0;             // hours enqueued so far (every active simulation is at this hour)
+  bool all_stopped = false;           // every simulation reported inactive
+  std::vector<uint32_t> rows_returned;
+  std::vector<SimState> state_h;
+
+  ~epi_sim() {
+    if (day_graph) (void)hipGraphExecDestroy(day_graph);
+    for (auto& e : poll_event)
+      if (e) (void)hipEventDestroy(e);
+    if (pinned_active) (void)hipHostFree(pinned_active);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int check_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(EPI_ERR_DEVICE, "no HIP device available (a gfx950 GPU is required)");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(EPI_ERR_DEVICE, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+  return EPI_OK;
+}
+
+int validate(const Config& c) {
+  const std::pair<const char*, double> ps[] = {
+      {"public_transport_percentage", c.public_transport_percentage},
+      {"working_percentage", c.working_percentage},
+      {"regular_transmission_rate", c.regular_transmission_rate},
+      {"high_transmission_rate", c.high_transmission_rate},
+      {"death_rate", c.death_rate},
+      {"percentage_asymptomatic_population", c.percentage_asymptomatic_population},
+      {"percentage_severe_infected_population", c.percentage_severe_infected_population},
+      {"hospital_beds_percentage", c.hospital_beds_percentage},
+      {"essential_workers_population", c.essential_workers_population}};
+  for (auto& p : ps)
+    if (!valid_p(p.second))
+      return fail(EPI_ERR_PANIC, std::string("called `Result::unwrap()` on an `Err` value: InvalidProbability (") +
+                                     p.first + "=" + std::to_string(p.second) + ")");
+  for (auto& v : c.vaccinate)
+    if (!valid_p(v.percent))
+      return fail(EPI_ERR_PANIC, "called `Result::unwrap()` on an `Err` value: InvalidProbability (Vaccinate.percent=" +
+                                     std::to_string(v.percent) + ")");
+  if (c.hours > 24u * kMaxDay) return fail(EPI_ERR_CONFIG, "hours above the supported maximum of 98280");
+  if (c.number_of_agents > (1u << 30)) return fail(EPI_ERR_CONFIG, "number_of_agents above 2^30");
+  return EPI_OK;
+}
+
+int build_launch_tables(epi_sim* s) {
+  const uint32_t B = s->shape.block;
+  s->agent_blocks.clear();
+  for (uint32_t i = 0; i < s->n_sims; ++i) {
+    const uint32_t n = s->desc[i].n;
+    const uint32_t nb = (n + B - 1) / B;
+    s->desc[i].n_blocks = nb;
+    s->desc[i].block_begin = static_cast<uint32_t>(s->agent_blocks.size());
+    for (uint32_t b = 0; b < nb; ++b) s->agent_blocks.push_back(BlockRef{i, b * B});
+  }
+  s->shape.agent_blocks = static_cast<uint32_t>(s->agent_blocks.size());
+  return EPI_OK;
+}
+
+// The device table is allocated for 64-thread blocks (the most blocks any shape needs).
+int upload_tables(epi_sim* s) {
+  if (s->shape.agent_blocks)
+    HIP_TRY(hipMemcpyAsync(s->d_agent_blocks, s->agent_blocks.data(), s->agent_blocks.size() * sizeof(BlockRef),
+                           hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->d_desc, s->desc.data(), s->desc.size() * sizeof(SimDesc), hipMemcpyHostToDevice,
+                         s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return EPI_OK;
+}
+
+int enqueue_hour(epi_sim* s, uint32_t next_hour) {
+  if (next_hour % 24u == 0) HIP_TRY(launch_hospital_scan(s->v, s->shape, s->stream));
+  HIP_TRY(launch_step(s->v, s->shape, s->stream));
+  HIP_TRY(launch_epilogue(s->v, s->shape, s->stream));
+  HIP_TRY(launch_commit(s->v, s->shape, s->stream));
+  return EPI_OK;
+}
+
+int capture_day_graph(epi_sim* s) {
+  if (s->day_graph) return EPI_OK;
+  hipGraph_t graph = nullptr;
+  HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = EPI_OK;
+  for (uint32_t k = 1; k <= 24 && rc == EPI_OK; ++k) rc = enqueue_hour(s, k);  // hours 24d+1 .. 24d+24
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (rc != EPI_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(EPI_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&s->day_graph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    s->day_graph = nullptr;
+    return fail(EPI_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return EPI_OK;
+}
+
+int fetch_states(epi_sim* s) {
+  HIP_TRY(hipMemcpyAsync(s->state_h.data(), s->v.state, s->n_sims * sizeof(SimState), hipMemcpyDeviceToHost,
+                         s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return EPI_OK;
+}
+
+bool all_inactive(const epi_sim* s, const uint32_t* flags) {
+  for (uint32_t i = 0; i < s->n_sims; ++i)
+    if (flags[i]) return false;
+  return true;
+}
+
+// Enqueue hours until `target` (or until every simulation has stopped). Whole days are
+// replayed from the captured graph; stop flags are polled one day behind the GPU.
+int advance(epi_sim* s, uint32_t target) {
+  if (s->any_empty && target > s->host_hour) return fail(EPI_ERR_PANIC, "No citizens!");
+  if (s->all_stopped) return EPI_OK;
+  const bool graph_ok = !(s->flags & EPI_FLAG_NO_GRAPH);
+  uint32_t day_index = 0;
+  bool pending[2] = {false, false};
+  while (s->host_hour < target) {
+    // poll the flags copied two days ago
+    const uint32_t slot = day_index & 1u;
+    if (pending[slot]) {
+      HIP_TRY(hipEventSynchronize(s->poll_event[slot]));
+      pending[slot] = false;
+      if (all_inactive(s, s->pinned_active + slot * s->n_sims)) break;
+    }
+    if (graph_ok && s->host_hour % 24u == 0 && target - s->host_hour >= 24u) {
+      int rc = capture_day_graph(s);
+      if (rc != EPI_OK) return rc;
+      HIP_TRY(hipGraphLaunch(s->day_graph, s->stream));
+      s->host_hour += 24;
+    } else {
+      const uint32_t end = std::min(target, (s->host_hour / 24u + 1u) * 24u);
+      while (s->host_hour < end) {
+        int rc = enqueue_hour(s, s->host_hour + 1);
+        if (rc != EPI_OK) return rc;
+        ++s->host_hour;
+      }
+    }
+    // copy the active flags of every simulation to pinned memory
+    for (uint32_t i = 0; i < s->n_sims; ++i)
+      HIP_TRY(hipMemcpyAsync(s->pinned_active + slot * s->n_sims + i, &s->v.state[i].active, sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipEventRecord(s->poll_event[slot], s->stream));
+    pending[slot] = true;
+    ++day_index;
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return EPI_OK;
+}
+
+int step_impl(epi_sim* s, uint32_t n_hours, epi_counts* rows, uint32_t row_stride, uint32_t* n_written,
+              int* stopped) {
+  const uint64_t want = static_cast<uint64_t>(s->host_hour) + n_hours;
+  const uint32_t target = static_cast<uint32_t>(std::min<uint64_t>(want, s->max_hour));
+  int rc = advance(s, target);
+  if (rc != EPI_OK) return rc;
+  rc = fetch_states(s);
+  if (rc != EPI_OK) return rc;
+  s->all_stopped = true;
+  for (uint32_t i = 0; i < s->n_sims; ++i) s->all_stopped = s->all_stopped && !s->state_h[i].active;
+  for (uint32_t i = 0; i < s->n_sims; ++i) {
+    const SimState& st = s->state_h[i];
+    if (st.error) return fail(EPI_ERR_PANIC, "device reported an error state");
+    const uint32_t from = s->rows_returned[i], to = st.rows_written;
+    const uint32_t k = to - from;
+    if (k > n_hours) return fail(EPI_ERR_DEVICE, "row accounting mismatch");
+    if (rows && k) {
+      HIP_TRY(hipMemcpy(rows + static_cast<size_t>(i) * row_stride, s->v.rows + s->desc[i].row_off + from,
+                        k * sizeof(Counts7), hipMemcpyDeviceToHost));
+    }
+    s->rows_returned[i] = to;
+    if (n_written) n_written[i] = k;
+    if (stopped) stopped[i] = st.active ? 0 : 1;
+  }
+  return EPI_OK;
+}
+
+int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_t>& seeds, uint32_t flags,
+                epi_sim** out) {
+  if (!out) return fail(EPI_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (texts.empty()) return fail(EPI_ERR_ARG, "no simulations");
+  std::unique_ptr<epi_sim> s(new epi_sim());
+  s->n_sims = static_cast<uint32_t>(texts.size());
+  s->flags = flags;
+  s->cfg.resize(s->n_sims);
+  s->seed = seeds;
+  for (uint32_t i = 0; i < s->n_sims; ++i) {
+    std::string err;
+    if (!parse_config(texts[i], s->cfg[i], err)) return fail(EPI_ERR_CONFIG, err);
+    int rc = validate(s->cfg[i]);
+    if (rc != EPI_OK) return rc;
+  }
+  int rc = check_device();
+  if (rc != EPI_OK) return rc;
+
+  // population synthesis (host), several simulations at a time
+  std::vector<Population> pops(s->n_sims);
+  std::vector<std::string> errs(s->n_sims);
+  std::vector<char> ok(s->n_sims, 0);
+  {
+    const unsigned nt = std::max(1u, std::min<unsigned>(s->n_sims, 8u));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+      th.emplace_back([&, t] {
+        for (uint32_t i = t; i < s->n_sims; i += nt)
+          ok[i] = synthesize_population(s->cfg[i], s->seed[i], pops[i], errs[i]) ? 1 : 0;
+      });
+    for (auto& x : th) x.join();
+  }
+  for (uint32_t i = 0; i < s->n_sims; ++i)
+    if (!ok[i]) return fail(EPI_ERR_PANIC, errs[i]);
+
+  // descriptors and offsets
+  s->desc.resize(s->n_sims);
+  s->geo.resize(s->n_sims);
+  s->counts0.resize(7 * s->n_sims);
+  s->rows_returned.assign(s->n_sims, 0);
+  s->state_h.resize(s->n_sims);
+  uint64_t agent_off = 0, cell_off = 0, row_off = 0, event_off = 0;
+  for (uint32_t i = 0; i < s->n_sims; ++i) {
+    const Config& c = s->cfg[i];
+    const Geometry& G = pops[i].geo;
+    s->geo[i] = G;
+    SimDesc& D = s->desc[i];
+    std::memset(&D, 0, sizeof D);
+    D.n = c.number_of_agents;
+    D.g = G.g;
+    D.stride = G.stride;
+    D.hours = c.hours;
+    D.agent_off = agent_off;
+    D.cell_off = cell_off;
+    D.row_off = row_off;
+    D.event_off = event_off;
+    D.event_cap = 4u + static_cast<uint32_t>(c.vaccinate.size()) + 2u * (c.hours / 24u + 1u);
+    D.seed_lo = static_cast<uint32_t>(s->seed[i]);
+    D.seed_hi = static_cast<uint32_t>(s->seed[i] >> 32);
+    D.housing = G.housing;
+    D.transport = G.transport;
+    D.work = G.work;
+    D.hosp[0] = G.hosp0;
+    D.hosp[1] = G.hosp1;
+    D.house_sx = G.house_sx;
+    D.house_sy = G.house_sy;
+    D.house_nx = std::max(G.house_nx, 1);
+    D.n_houses = std::max(G.n_houses, 1u);
+    D.office_sx = G.office_sx;
+    D.office_sy = G.office_sy;
+    D.office_nx = std::max(G.office_nx, 1);
+    D.n_offices = std::max(G.n_offices, 1u);
+    D.regular_start = c.regular_transmission_start_day;
+    D.high_start = c.high_transmission_start_day;
+    D.last_day = c.last_day;
+    D.exposed_duration = c.exposed_duration;
+    D.pre_symptomatic_duration = c.pre_symptomatic_duration;
+    D.thr_regular = bernoulli_threshold(c.regular_transmission_rate);
+    D.thr_high = bernoulli_threshold(c.high_transmission_rate);
+    D.thr_death = bernoulli_threshold(c.death_rate);
+    D.thr_symptoms = bernoulli_threshold(1.0 - c.percentage_asymptomatic_population);
+    D.thr_severe = bernoulli_threshold(c.percentage_severe_infected_population);
+    D.hosp_window = hospital_window_bits(c);
+    D.has_lockdown = c.has_lockdown;
+    D.lockdown_at = c.lockdown_at_number_of_infections;
+    D.has_hospital = c.has_build_new_hospital;
+    D.spread_rate_threshold = c.spread_rate_threshold;
+    // vaccination schedule: HashMap<hour, percent> semantics (vaccination.rs:37-50)
+    std::vector<std::pair<uint32_t, double>> sched;
+    for (auto& v : c.vaccinate) {
+      auto it = std::find_if(sched.begin(), sched.end(), [&](auto& e) { return e.first == v.at_hour; });
+      if (it != sched.end()) it->second = v.percent; else sched.emplace_back(v.at_hour, v.percent);
+    }
+    std::sort(sched.begin(), sched.end());
+    D.vacc_off = static_cast<uint32_t>(s->vacc_hour.size());
+    D.n_vacc = static_cast<uint32_t>(sched.size());
+    for (auto& e : sched) {
+      s->vacc_hour.push_back(e.first);
+      s->vacc_thr.push_back(bernoulli_threshold(e.second));
+    }
+    for (const Rect* r : {&G.housing, &G.transport, &G.work, &G.hosp0, &G.hosp1})
+      if (!rect_inside(*r, G.stride, G.height))
+        return fail(EPI_ERR_CONFIG, "grid_size too small for the engine's area layout");
+    if (c.number_of_agents == 0 && c.hours > 1) s->any_empty = true;
+    agent_off += D.n;
+    cell_off += static_cast<uint64_t>(G.stride) * G.height;
+    row_off += c.hours > 1 ? c.hours - 1 : 0;
+    event_off += D.event_cap;
+    s->max_hour = std::max(s->max_hour, c.hours > 1 ? c.hours - 1 : 0u);
+    std::memcpy(&s->counts0[7 * i], pops[i].counts0, sizeof(pops[i].counts0));
+    const auto cells = [](const Rect& r) {
+      return static_cast<uint32_t>(r.x1 - r.x0 + 1) * static_cast<uint32_t>(r.y1 - r.y0 + 1);
+    };
+    const uint32_t hosp_cells = std::max(cells(G.hosp0), cells(G.hosp1));
+    const uint32_t per_block = s->shape.hosp_block * s->shape.hosp_cells_per_thread;
+    for (uint32_t first = 0; first < hosp_cells; first += per_block)
+      s->hosp_blocks.push_back(BlockRef{i, first});
+  }
+  s->n_agents = agent_off;
+  s->n_cells = cell_off;
+  s->n_rows = row_off;
+  s->n_events = event_off;
+  s->shape.hosp_blocks = static_cast<uint32_t>(s->hosp_blocks.size());
+  s->shape.n_sims = s->n_sims;
+  rc = build_launch_tables(s.get());
+  if (rc != EPI_OK) return rc;
+
+  // device memory
+  HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  DeviceBuffers& M = s->mem;
+  DevView& v = s->v;
+  HIP_TRY(M.alloc(&v.pos, s->n_agents));
+  HIP_TRY(M.alloc(&v.st, s->n_agents));
+  HIP_TRY(M.alloc(&v.tm, s->n_agents));
+  HIP_TRY(M.alloc(&v.wsa, s->n_agents));
+  HIP_TRY(M.alloc(&v.tgt, s->n_agents));
+  HIP_TRY(M.alloc(&v.occ, s->n_cells));
+  HIP_TRY(M.alloc(&v.claim, s->n_cells));
+  HIP_TRY(M.alloc(&s->d_desc, s->n_sims));
+  v.desc = s->d_desc;
+  HIP_TRY(M.alloc(&v.state, s->n_sims));
+  // table sized for the smallest legal block (64 threads) so block-size changes never reallocate
+  uint64_t max_blocks = 0;
+  for (uint32_t i = 0; i < s->n_sims; ++i) max_blocks += (s->desc[i].n + 63) / 64;
+  HIP_TRY(M.alloc(&s->d_agent_blocks, max_blocks));
+  HIP_TRY(M.alloc(&v.block_counts, 6 * max_blocks));
+  v.agent_blocks = s->d_agent_blocks;
+  BlockRef* d_hosp_blocks = nullptr;
+  HIP_TRY(M.alloc(&d_hosp_blocks, s->hosp_blocks.size()));
+  v.hosp_blocks = d_hosp_blocks;
+  HIP_TRY(M.alloc(&v.rows, s->n_rows));
+  HIP_TRY(M.alloc(&v.events, s->n_events));
+  uint32_t* d_vacc_hour = nullptr;
+  unsigned long long* d_vacc_thr = nullptr;
+  HIP_TRY(M.alloc(&d_vacc_hour, s->vacc_hour.size()));
+  HIP_TRY(M.alloc(&d_vacc_thr, s->vacc_thr.size()));
+  v.vacc_hour = d_vacc_hour;
+  v.vacc_thr = d_vacc_thr;
+
+  std::vector<uint8_t> grid;
+  for (uint32_t i = 0; i < s->n_sims; ++i) {
+    const SimDesc& D = s->desc[i];
+    const Population& P = pops[i];
+    const size_t nb = static_cast<size_t>(D.n) * sizeof(uint32_t);
+    if (D.n) {
+      HIP_TRY(hipMemcpy(v.pos + D.agent_off, P.pos.data(), nb, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(v.st + D.agent_off, P.st.data(), nb, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(v.tm + D.agent_off, P.tm.data(), nb, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(v.wsa + D.agent_off, P.wsa.data(), nb, hipMemcpyHostToDevice));
+    }
+    const size_t cells = static_cast<size_t>(P.geo.stride) * P.geo.height;
+    grid.assign(cells, 0);
+    for (uint32_t a = 0; a < D.n; ++a) {
+      const uint32_t p = P.pos[a];
+      grid[static_cast<size_t>(p >> 16) * P.geo.stride + (p & 0xFFFFu)] = host_grid_byte(s->cfg[i], P.st[a]);
+    }
+    HIP_TRY(hipMemcpy(v.occ + D.cell_off, grid.data(), cells, hipMemcpyHostToDevice));
+    SimState st;
+    std::memset(&st, 0, sizeof st);
+    st.active = (s->cfg[i].hours > 1 && D.n > 0) ? 1u : 0u;
+    st.first_vacant = ~0u;
+    s->state_h[i] = st;
+  }
+  HIP_TRY(hipMemcpy(v.state, s->state_h.data(), s->n_sims * sizeof(SimState), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_hosp_blocks, s->hosp_blocks.data(), s->hosp_blocks.size() * sizeof(BlockRef),
+                    hipMemcpyHostToDevice));
+  if (!s->vacc_hour.empty()) {
+    HIP_TRY(hipMemcpy(d_vacc_hour, s->vacc_hour.data(), s->vacc_hour.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_vacc_thr, s->vacc_thr.data(), s->vacc_thr.size() * sizeof(unsigned long long),
+                      hipMemcpyHostToDevice));
+  }
+  HIP_TRY(launch_fill_u64(v.claim, s->n_cells, ~0ull, s->stream));
+  rc = upload_tables(s.get());
+  if (rc != EPI_OK) return rc;
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_active), 2 * s->n_sims * sizeof(uint32_t),
+                        hipHostMallocDefault));
+  HIP_TRY(hipEventCreateWithFlags(&s->poll_event[0], hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&s->poll_event[1], hipEventDisableTiming));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  *out = s.release();
+  return EPI_OK;
+}
+
+std::string utc_stamp() {
+  std::time_t t = std::time(nullptr);
+  std::tm tm{};
+  gmtime_r(&t, &tm);
+  char buf[32];
+  std::strftime(buf, sizeof buf, "%Y-%m-%dT%H:%M:%S", &tm);
+  return buf;
+}
+
+// EPI_OUTPUT_DIR prefixing of a relative listener path (engine/src/utils/environment.rs:33-38)
+std::string listener_path(const std::string& name) {
+  if (!name.empty() && name[0] == '/') return name;
+  const char* env = std::getenv("EPI_OUTPUT_DIR");
+  std::string base = env ? env : ".";
+  if (!base.empty() && base.back() != '/') base += '/';
+  return base + name;
+}
+
+const char* event_name(uint32_t kind) {
+  return kind == EV_VACCINATION ? "vaccination" : (kind == EV_HOSPITAL ? "build_new_hospital" : "lockdown");
+}
+const char* event_data(uint32_t kind) {
+  if (kind == EV_LOCKED) return "{\"status\":\"locked_down\"}";
+  if (kind == EV_UNLOCKED) return "{\"status\":\"lockdown_revoked\"}";
+  return "{}";
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* epi_last_error(void) { return g_last_error.c_str(); }
+
+int epi_create(const char* config_json, uint64_t seed, uint32_t flags, epi_sim** out) {
+  if (!config_json) return fail(EPI_ERR_ARG, "config_json is null");
+  g_last_error.clear();
+  return create_impl({std::string(config_json)}, {seed}, flags, out);
+}
+
+int epi_batch_create(const char* const* configs, const uint64_t* seeds, uint32_t n, uint32_t flags, epi_sim** out) {
+  if (!configs || !seeds || n == 0) return fail(EPI_ERR_ARG, "configs/seeds null or n == 0");
+  g_last_error.clear();
+  std::vector<std::string> texts;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!configs[i]) return fail(EPI_ERR_ARG, "configs[" + std::to_string(i) + "] is null");
+    texts.emplace_back(configs[i]);
+  }
+  return create_impl(texts, std::vector<uint64_t>(seeds, seeds + n), flags, out);
+}
+
+void epi_destroy(epi_sim* sim) { delete sim; }
+
+int epi_step(epi_sim* sim, uint32_t n_hours, epi_counts* rows, uint32_t* n_written, int* stopped) {
+  if (!sim) return fail(EPI_ERR_ARG, "sim is null");
+  if (sim->n_sims != 1) return fail(EPI_ERR_ARG, "batch handle: use epi_batch_step");
+  return step_impl(sim, n_hours, rows, n_hours, n_written, stopped);
+}
+
+int epi_batch_step(epi_sim* sim, uint32_t n_hours, epi_counts* rows, uint32_t* n_written, int* stopped) {
+  if (!sim) return fail(EPI_ERR_ARG, "sim is null");
+  return step_impl(sim, n_hours, rows, n_hours, n_written, stopped);
+}
+
+int epi_info(epi_sim* sim, uint32_t index, uint32_t* n_sims, uint32_t* n_agents, uint32_t* grid_size,
+             uint32_t* hours) {
+  if (!sim || index >= sim->n_sims) return fail(EPI_ERR_ARG, "bad handle or index");
+  if (n_sims) *n_sims = sim->n_sims;
+  if (n_agents) *n_agents = sim->desc[index].n;
+  if (grid_size) *grid_size = sim->desc[index].g;
+  if (hours) *hours = sim->desc[index].hours;
+  return EPI_OK;
+}
+
+int epi_geometry(epi_sim* sim, uint32_t index, int32_t rects[20], uint32_t* hospital_version) {
+  if (!sim || index >= sim->n_sims || !rects) return fail(EPI_ERR_ARG, "bad handle, index or rects");
+  const Geometry& G = sim->geo[index];
+  const Rect rs[5] = {G.housing, G.transport, G.work, G.hosp0, G.hosp1};
+  for (int k = 0; k < 5; ++k) {
+    rects[4 * k] = rs[k].x0;
+    rects[4 * k + 1] = rs[k].y0;
+    rects[4 * k + 2] = rs[k].x1;
+    rects[4 * k + 3] = rs[k].y1;
+  }
+  if (hospital_version) {
+    int rc = fetch_states(sim);
+    if (rc != EPI_OK) return rc;
+    *hospital_version = sim->state_h[index].hosp_version;
+  }
+  return EPI_OK;
+}
+
+int epi_current_counts(epi_sim* sim, uint32_t index, epi_counts* out) {
+  if (!sim || index >= sim->n_sims || !out) return fail(EPI_ERR_ARG, "bad handle, index or out");
+  int rc = fetch_states(sim);
+  if (rc != EPI_OK) return rc;
+  const SimState& st = sim->state_h[index];
+  if (st.rows_written == 0) {
+    const uint32_t* c = &sim->counts0[7 * index];
+    *out = epi_counts{c[0], c[1], c[2], c[3], c[4], c[5], c[6]};
+    return EPI_OK;
+  }
+  Counts7 row;
+  HIP_TRY(hipMemcpy(&row, sim->v.rows + sim->desc[index].row_off + st.rows_written - 1, sizeof row,
+                    hipMemcpyDeviceToHost));
+  *out = epi_counts{row.hour, row.s, row.e, row.i, row.h, row.r, row.d};
+  return EPI_OK;
+}
+
+int epi_events(epi_sim* sim, uint32_t index, epi_event* out, uint32_t cap, uint32_t* n) {
+  if (!sim || index >= sim->n_sims) return fail(EPI_ERR_ARG, "bad handle or index");
+  int rc = fetch_states(sim);
+  if (rc != EPI_OK) return rc;
+  const SimDesc& D = sim->desc[index];
+  const uint32_t total = sim->state_h[index].n_events;
+  const uint32_t avail = std::min(total, D.event_cap);
+  if (total > D.event_cap) return fail(EPI_ERR_DEVICE, "event log overflow");
+  if (n) *n = total;
+  const uint32_t k = std::min(avail, cap);
+  if (out && k) HIP_TRY(hipMemcpy(out, sim->v.events + D.event_off, k * sizeof(Event), hipMemcpyDeviceToHost));
+  return EPI_OK;
+}
+
+int epi_read_agents(epi_sim* sim, uint32_t index, epi_agents_soa* dst) {
+  if (!sim || index >= sim->n_sims || !dst) return fail(EPI_ERR_ARG, "bad handle, index or dst");
+  const SimDesc& D = sim->desc[index];
+  const Geometry& G = sim->geo[index];
+  const uint32_t n = D.n;
+  std::vector<uint32_t> pos(n), st(n), tm(n), wsa(n);
+  if (n) {
+    HIP_TRY(hipMemcpy(pos.data(), sim->v.pos + D.agent_off, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(st.data(), sim->v.st + D.agent_off, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(tm.data(), sim->v.tm + D.agent_off, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(wsa.data(), sim->v.wsa + D.agent_off, n * 4, hipMemcpyDeviceToHost));
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t w = st[i];
+    const uint32_t state = st_state(w), sev = st_sev(w), ws = st_ws(w);
+    if (dst->x) dst->x[i] = static_cast<int32_t>(pos[i] & 0xFFFFu);
+    if (dst->y) dst->y[i] = static_cast<int32_t>(pos[i] >> 16);
+    if (dst->state) dst->state[i] = static_cast<uint8_t>(state);
+    if (dst->severity) dst->severity[i] = static_cast<uint8_t>(state == ST_I ? sev : 0);
+    if (dst->infection_day) dst->infection_day[i] = state == ST_I ? st_day(w) : 0;
+    if (dst->at_hour) dst->at_hour[i] = (state == ST_E || (state == ST_I && sev == SEV_PRE)) ? tm[i] : 0;
+    if (dst->flags)
+      dst->flags[i] = static_cast<uint8_t>(((w & F_HOSP) ? 1 : 0) | ((w & F_VACC) ? 2 : 0) | ((w & F_ISO) ? 4 : 0) |
+                                           ((w & F_WQ) ? 8 : 0) | ((w & F_PT) ? 16 : 0));
+    if (dst->work_status) dst->work_status[i] = static_cast<uint8_t>(ws);
+    if (dst->work_start_at) dst->work_start_at[i] = ws == WS_STAFF ? wsa[i] : 0;
+    if (dst->immunity) dst->immunity[i] = static_cast<int8_t>(st_imm(w));
+    if (dst->current_area) {
+      const Rect home = building_rect(G.house_sx, G.house_sy, std::max(G.house_nx, 1), 2, i % std::max(G.n_houses, 1u));
+      Rect r{};
+      switch (st_area(w)) {
+        case AREA_HOME: r = home; break;
+        case AREA_WORK:
+          r = ws == WS_NA ? home
+                          : building_rect(G.office_sx, G.office_sy, std::max(G.office_nx, 1), 10,
+                                          i % std::max(G.n_offices, 1u));
+          break;
+        case AREA_TRANSPORT: r = G.transport; break;
+        case AREA_HOUSING: r = G.housing; break;
+        case AREA_HOSP0: r = G.hosp0; break;
+        default: r = G.hosp1; break;
+      }
+      dst->current_area[4 * i] = r.x0;
+      dst->current_area[4 * i + 1] = r.y0;
+      dst->current_area[4 * i + 2] = r.x1;
+      dst->current_area[4 * i + 3] = r.y1;
+    }
+  }
+  return EPI_OK;
+}
+
+int epi_run(epi_sim* sim, const char* output_dir, char* final_path, size_t cap) {
+  if (!sim || !output_dir) return fail(EPI_ERR_ARG, "sim or output_dir is null");
+  if (sim->n_sims != 1) return fail(EPI_ERR_ARG, "epi_run drives a single simulation");
+  // create_out_dir_if_not_present (engine/src/utils/util.rs:28-35): errors ignored
+  std::string dir(output_dir);
+  std::string out_dir = dir + (dir.empty() || dir.back() == '/' ? "" : "/") + "output";
+  (void)::mkdir(out_dir.c_str(), 0777);
+  const std::string base = out_dir + "/simulation_0_" + utc_stamp();
+  std::vector<epi_counts> rows;
+  const uint32_t chunk = 24u * 50u;
+  std::vector<epi_counts> buf(chunk);
+  for (;;) {
+    uint32_t nw = 0;
+    int stopped = 0;
+    int rc = step_impl(sim, chunk, buf.data(), chunk, &nw, &stopped);
+    if (rc != EPI_OK) return rc;
+    rows.insert(rows.end(), buf.begin(), buf.begin() + nw);
+    if (stopped || nw == 0) break;
+  }
+  // CSV (listeners/csv_service.rs:44-71): serde field order of Counts, one row per hour
+  const std::string csv_path = listener_path(base + ".csv");
+  {
+    std::ofstream f(csv_path);
+    if (!f) return fail(EPI_ERR_IO, "Failed to write to file: " + csv_path);
+    f << "hour,susceptible,exposed,infected,hospitalized,recovered,deceased\n";
+    for (const auto& r : rows)
+      f << r.hour << ',' << r.susceptible << ',' << r.exposed << ',' << r.infected << ',' << r.hospitalized << ','
+        << r.recovered << ',' << r.deceased << '\n';
+    if (!f) return fail(EPI_ERR_IO, "Failed to write to file: " + csv_path);
+  }
+  // interventions report (listeners/intervention_reporter.rs:53-58): compact serde_json
+  uint32_t n_ev = 0;
+  int rc = epi_events(sim, 0, nullptr, 0, &n_ev);
+  if (rc != EPI_OK) return rc;
+  std::vector<epi_event> ev(n_ev);
+  if (n_ev) {
+    rc = epi_events(sim, 0, ev.data(), n_ev, &n_ev);
+    if (rc != EPI_OK) return rc;
+  }
+  const std::string json_path = listener_path(base + "_interventions.json");
+  {
+    std::ofstream f(json_path);
+    if (!f) return fail(EPI_ERR_IO, "Failed to create intervention report file: " + json_path);
+    f << '[';
+    for (size_t k = 0; k < ev.size(); ++k) {
+      if (k) f << ',';
+      f << "{\"hour\":" << ev[k].hour << ",\"intervention\":\"" << event_name(ev[k].kind) << "\",\"data\":"
+        << event_data(ev[k].kind) << '}';
+    }
+    f << ']';
+  }
+  if (final_path && cap) std::snprintf(final_path, cap, "%s", csv_path.c_str());
+  return EPI_OK;
+}
+
+int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t launches[3], uint32_t* hours_run) {
+  if (!sim || !ms_mean || !launches) return fail(EPI_ERR_ARG, "null argument");
+  if (sim->any_empty) return fail(EPI_ERR_PANIC, "No citizens!");
+  const uint32_t target = std::min<uint64_t>(static_cast<uint64_t>(sim->host_hour) + n_hours, sim->max_hour);
+  double sum[3] = {0, 0, 0};
+  uint32_t cnt[3] = {0, 0, 0};
+  const uint32_t per_batch = 24;
+  std::vector<hipEvent_t> ev(4 * per_batch, nullptr);
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  uint32_t ran = 0;
+  int rc = EPI_OK;
+  while (sim->host_hour < target && rc == EPI_OK) {
+    const uint32_t nb = std::min(per_batch, target - sim->host_hour);
+    std::vector<int> had_scan(nb, 0);
+    for (uint32_t k = 0; k < nb && rc == EPI_OK; ++k) {
+      const uint32_t h = sim->host_hour + 1;
+      hipEvent_t* e = &ev[4 * k];
+      (void)hipEventRecord(e[0], sim->stream);
+      if (h % 24u == 0) {
+        had_scan[k] = 1;
+        if (launch_hospital_scan(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      }
+      (void)hipEventRecord(e[1], sim->stream);
+      if (launch_step(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      if (launch_epilogue(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      (void)hipEventRecord(e[2], sim->stream);
+      if (launch_commit(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      (void)hipEventRecord(e[3], sim->stream);
+      ++sim->host_hour;
+    }
+    if (hipStreamSynchronize(sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "hipStreamSynchronize");
+    for (uint32_t k = 0; k < nb && rc == EPI_OK; ++k) {
+      float t[3] = {0, 0, 0};
+      hipEvent_t* e = &ev[4 * k];
+      (void)hipEventElapsedTime(&t[0], e[0], e[1]);
+      (void)hipEventElapsedTime(&t[1], e[1], e[2]);
+      (void)hipEventElapsedTime(&t[2], e[2], e[3]);
+      if (had_scan[k]) { sum[0] += t[0]; ++cnt[0]; }
+      sum[1] += t[1]; ++cnt[1];
+      sum[2] += t[2]; ++cnt[2];
+    }
+    ran += nb;
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (rc != EPI_OK) return rc;
+  for (int k = 0; k < 3; ++k) {
+    ms_mean[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+    launches[k] = cnt[k];
+  }
+  if (hours_run) *hours_run = ran;
+  // keep row accounting consistent for later epi_step calls
+  rc = fetch_states(sim);
+  if (rc != EPI_OK) return rc;
+  for (uint32_t i = 0; i < sim->n_sims; ++i) sim->rows_returned[i] = sim->state_h[i].rows_written;
+  return EPI_OK;
+}
+
+int epi_set_block_size(epi_sim* sim, uint32_t threads) {
+  if (!sim) return fail(EPI_ERR_ARG, "sim is null");
+  if (threads < 64 || threads > 1024 || threads % 64) return fail(EPI_ERR_ARG, "block size must be 64..1024, x64");
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  sim->shape.block = threads;
+  int rc = build_launch_tables(sim);
+  if (rc != EPI_OK) return rc;
+  rc = upload_tables(sim);
+  if (rc != EPI_OK) return rc;
+  if (sim->day_graph) {
+    (void)hipGraphExecDestroy(sim->day_graph);
+    sim->day_graph = nullptr;
+  }
+  return EPI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,600 @@
+// Hour-step kernels for gfx950.
+//
+// One hour of EpiRust's standalone engine (engine/src/allocation_map.rs:67-134 and
+// :306-337, engine/src/citizen/mod.rs:227-454, default_disease_handler.rs:31-103) is
+//   [hour-of-day 0 only] k_hospital_scan : first vacant hospital cell of the snapshot
+//   k_step   : per agent, routine + disease update against the start-of-hour grid;
+//              state written in place (it is read by no other agent), the chosen cell
+//              claimed with a 64-bit atomicMin keyed (hour, agent id); S/E/I/H/R/D
+//              counted with wavefront ballots into per-block partials.
+//   k_epilogue: one block per simulation reduces the partials, writes the hour's row and
+//              runs the intervention triggers and the stop rule on device.
+//   k_commit : per agent, the claim winner moves, grid bytes are updated, and the
+//              vaccination / lockdown sweeps decided by k_step's epilogue are applied.
+// Every kernel is batched: blocks map to (simulation, agent range) through a table, so
+// many independent replicates share one launch. Nothing here uses MFMA: the step is
+// integer work bound by HBM traffic.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "philox.hpp"
+
+namespace epi {
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ uint32_t first_lane(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ bool contains(const Rect& a, int32_t x, int32_t y) {
+  return a.x0 <= x && a.x1 >= x && a.y0 <= y && a.y1 >= y;
+}
+__device__ __forceinline__ bool same_rect(const Rect& a, const Rect& b) {
+  return a.x0 == b.x0 && a.y0 == b.y0 && a.x1 == b.x1 && a.y1 == b.y1;
+}
+__device__ __forceinline__ Rect building(int32_t sx, int32_t sy, int32_t nx, int32_t size, uint32_t k) {
+  const int32_t x0 = sx + static_cast<int32_t>(k % static_cast<uint32_t>(nx)) * size;
+  const int32_t y0 = sy + static_cast<int32_t>(k / static_cast<uint32_t>(nx)) * size;
+  return Rect{x0, y0, x0 + size - 1, y0 + size - 1};
+}
+
+__constant__ int8_t kMooreDx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};  // point.rs:57-61
+__constant__ int8_t kMooreDy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+
+__device__ __forceinline__ uint32_t fld(uint32_t w, uint32_t b, uint32_t m) { return (w >> b) & m; }
+__device__ __forceinline__ uint32_t set_fld(uint32_t w, uint32_t b, uint32_t m, uint32_t v) {
+  return (w & ~(m << b)) | ((v & m) << b);
+}
+
+// Transmission window of (infection_day + immunity) as u32 (citizen/mod.rs:182-185,
+// common/src/disease/mod.rs:88-95): 1 regular, 2 high, 0 rate 0.0.
+__device__ __forceinline__ uint32_t window_of(const SimDesc& D, uint32_t w) {
+  const uint32_t day = fld(w, B_STATE, M_STATE) == ST_I ? fld(w, B_DAY, M_DAY) : 0u;
+  const uint32_t d = static_cast<uint32_t>(static_cast<int32_t>(day) + static_cast<int32_t>(fld(w, B_IMM, M_IMM)) - 2);
+  if (D.regular_start < d && d <= D.high_start) return 1u;
+  if (D.high_start < d && d <= D.last_day) return 2u;
+  return 0u;
+}
+
+// Grid byte of an occupied cell: 1 + (infectious window or 0).
+__device__ __forceinline__ uint8_t grid_byte(const SimDesc& D, uint32_t w) {
+  const bool infectious = fld(w, B_STATE, M_STATE) == ST_I && !(w & F_HOSP);
+  return static_cast<uint8_t>(1u + (infectious ? window_of(D, w) : 0u));
+}
+
+__device__ __forceinline__ unsigned long long claim_key(uint32_t hour, uint32_t id) {
+  return (static_cast<unsigned long long>(0xFFFFFFFFu - hour) << 32) | id;
+}
+
+// Per-thread state of one agent's hour.
+struct Agent {
+  const SimDesc& D;
+  const SimState& S;
+  const uint8_t* __restrict__ occ;  // this simulation's grid
+  uint32_t id, h, w, tm, wsa;
+  int32_t x, y;                     // current cell
+  int32_t tx, ty;                   // chosen cell
+  bool have_mv = false, have_dz = false;
+  int inf_block = -1;
+  u32x4 mv, dz, inf;
+
+  __device__ Agent(const SimDesc& d, const SimState& s, const uint8_t* o) : D(d), S(s), occ(o) {}
+
+  __device__ __forceinline__ u32x4 dr(uint32_t site, uint32_t block) const {
+    return philox4x32_10(id, h, site, block, D.seed_lo, D.seed_hi);
+  }
+  __device__ __forceinline__ const u32x4& move_words() {
+    if (!have_mv) { mv = dr(SITE_MOVE, 0); have_mv = true; }
+    return mv;
+  }
+  __device__ __forceinline__ const u32x4& disease_words() {
+    if (!have_dz) { dz = dr(SITE_DISEASE, 0); have_dz = true; }
+    return dz;
+  }
+  __device__ __forceinline__ uint64_t infect_word(int k) {
+    const int b = k >> 1;
+    if (inf_block != b) { inf = dr(SITE_INFECT, static_cast<uint32_t>(b)); inf_block = b; }
+    const int i = 2 * (k & 1);
+    return join64(inf.w[i], inf.w[i + 1]);
+  }
+
+  __device__ __forceinline__ uint32_t state() const { return fld(w, B_STATE, M_STATE); }
+  __device__ __forceinline__ uint32_t sev() const { return fld(w, B_SEV, M_SEV); }
+  __device__ __forceinline__ uint32_t ws() const { return fld(w, B_WS, M_WS); }
+  __device__ __forceinline__ bool symptomatic() const {
+    return state() == ST_I && (sev() == SEV_MILD || sev() == SEV_SEVERE);
+  }
+  __device__ __forceinline__ bool can_move() const {  // citizen/mod.rs:452-454
+    return !(symptomatic() || (w & F_HOSP) || state() == ST_D || (w & F_ISO));
+  }
+  __device__ __forceinline__ Rect home() const {
+    return building(D.house_sx, D.house_sy, D.house_nx, 2, id % D.n_houses);
+  }
+  __device__ __forceinline__ Rect work() const {  // non-workers' work location is home
+    return ws() == WS_NA ? home() : building(D.office_sx, D.office_sy, D.office_nx, 10, id % D.n_offices);
+  }
+  __device__ __forceinline__ Rect area_rect(uint32_t tag) const {
+    switch (tag) {
+      case AREA_HOME: return home();
+      case AREA_WORK: return work();
+      case AREA_TRANSPORT: return D.transport;
+      case AREA_HOUSING: return D.housing;
+      case AREA_HOSP0: return D.hosp[0];
+      default: return D.hosp[1];
+    }
+  }
+  __device__ __forceinline__ Rect current_area() const { return area_rect(fld(w, B_AREA, M_AREA)); }
+  __device__ __forceinline__ void set_area(uint32_t tag) { w = set_fld(w, B_AREA, M_AREA, tag); }
+
+  __device__ __forceinline__ bool vacant(int32_t cx, int32_t cy) const {
+    return occ[static_cast<uint32_t>(cy) * D.stride + static_cast<uint32_t>(cx)] == 0;
+  }
+  __device__ __forceinline__ bool in_grid(int32_t cx, int32_t cy) const {
+    return cx >= 0 && cy >= 0 && cx < static_cast<int32_t>(D.g) && cy < static_cast<int32_t>(D.g);
+  }
+  __device__ __forceinline__ void random_point(const Rect& a, int32_t& px, int32_t& py) {  // area.rs:76-81
+    const u32x4& m = move_words();
+    px = a.x0 + static_cast<int32_t>(below(m.w[0], static_cast<uint32_t>(a.x1 - a.x0 + 1)));
+    py = a.y0 + static_cast<int32_t>(below(m.w[1], static_cast<uint32_t>(a.y1 - a.y0 + 1)));
+  }
+  // move_agent (allocation_map.rs:136-142): the target if vacant in the snapshot, else stay
+  __device__ __forceinline__ void move_to(int32_t px, int32_t py) {
+    if (vacant(px, py)) { tx = px; ty = py; }
+  }
+
+  // citizen/mod.rs:415-432
+  __device__ void move_agent_from() {
+    if (!can_move()) return;
+    const Rect a = current_area();
+    int32_t lx = x, ly = y;
+    if (!contains(a, x, y)) random_point(a, lx, ly);  // teleport origin, vacancy not checked
+    uint32_t cand = 0, n = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int32_t cx = lx + kMooreDx[k], cy = ly + kMooreDy[k];
+      if (contains(a, cx, cy) && in_grid(cx, cy) && vacant(cx, cy)) { cand |= 1u << k; ++n; }
+    }
+    if (n == 0) return;
+    uint32_t pick = below(move_words().w[2], n);
+    // index of the pick-th set bit of cand
+    for (uint32_t j = 0; j < pick; ++j) cand &= cand - 1;
+    const int k = __builtin_ctz(cand);
+    tx = lx + kMooreDx[k];
+    ty = ly + kMooreDy[k];
+  }
+
+  // citizen/mod.rs:367-395
+  __device__ void goto_area(const Rect& target) {
+    bool override_movement = false;
+    const uint32_t s = ws();
+    if (s == WS_NORMAL || s == WS_ESSENTIAL) {
+      const Rect wk = work();
+      if (contains(wk, x, y) && same_rect(target, home()) && symptomatic()) override_movement = true;
+    }
+    if (!can_move() && !override_movement) return;
+    if (s != WS_NA) {
+      int32_t px, py;
+      random_point(target, px, py);
+      move_to(px, py);
+      return;
+    }
+    move_agent_from();
+  }
+
+  // default_disease_handler.rs:64-86 (OR over fixed per-neighbour draws)
+  __device__ bool on_susceptible() {
+    if ((w & F_WQ) || (w & F_VACC)) return false;
+    const Rect a = current_area();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int32_t cx = tx + kMooreDx[k], cy = ty + kMooreDy[k];
+      if (!contains(a, cx, cy) || !in_grid(cx, cy)) continue;
+      const uint8_t b = occ[static_cast<uint32_t>(cy) * D.stride + static_cast<uint32_t>(cx)];
+      if (b < 2) continue;
+      const uint64_t thr = b == 2 ? D.thr_regular : D.thr_high;
+      if (bernoulli(thr, infect_word(k))) return true;
+    }
+    return false;
+  }
+
+  // disease_state_machine.rs:53-70 + default_disease_handler.rs:41-62
+  __device__ void update_infection_dynamics(const uint32_t* __restrict__ tm_arr, uint64_t gi) {
+    const uint32_t s = state();
+    if (s == ST_S) {
+      if (on_susceptible()) {
+        w = set_fld(w, B_STATE, M_STATE, ST_E);
+        tm = h;
+      }
+    } else if (s == ST_E) {
+      tm = tm_arr[gi];
+      const u32x4& d = disease_words();
+      const int32_t rf = static_cast<int32_t>(below(d.w[0], 3)) - 1;  // RANGE_FOR_EXPOSED
+      if (h - tm >= static_cast<uint32_t>(static_cast<int32_t>(D.exposed_duration) + rf)) {
+        const bool symptoms = bernoulli(D.thr_symptoms, join64(d.w[2], d.w[3]));
+        w = set_fld(w, B_STATE, M_STATE, ST_I);
+        w = set_fld(w, B_DAY, M_DAY, 0);
+        w = set_fld(w, B_SEV, M_SEV, symptoms ? SEV_PRE : SEV_ASYM);
+        tm = symptoms ? h : 0u;
+      }
+    } else if (s == ST_I && sev() == SEV_PRE) {
+      tm = tm_arr[gi];
+      if (h - tm >= D.pre_symptomatic_duration) {
+        const u32x4& d = disease_words();
+        w = set_fld(w, B_SEV, M_SEV, bernoulli(D.thr_severe, join64(d.w[2], d.w[3])) ? SEV_SEVERE : SEV_MILD);
+        tm = 0;
+      }
+    }
+  }
+
+  // citizen/mod.rs:257-349
+  __device__ void perform_movements(uint32_t hd, const uint32_t* __restrict__ tm_arr, uint64_t gi) {
+    const uint32_t s = ws();
+    if (s == WS_NORMAL || s == WS_ESSENTIAL) {
+      if (hd == 7 || hd == 17) {
+        if (w & F_PT) { goto_area(D.transport); set_area(AREA_TRANSPORT); }
+        else move_agent_from();
+      } else if (hd == 8) {
+        goto_area(work());
+        set_area(AREA_WORK);
+      } else if (hd == 16) {
+        goto_area(home());
+        set_area(AREA_HOME);
+      } else {
+        move_agent_from();
+      }
+    } else if (s == WS_STAFF) {
+      const uint32_t since = h > wsa ? h - wsa : 0u;  // saturating_sub
+      if (since == 24u * 14u) { w |= F_WQ; return; }
+      if (since == 24u * 14u * 2u) {
+        goto_area(home());
+        set_area(AREA_HOME);
+        wsa = h + 24u * 14u;
+        return;
+      }
+      if (hd == 8) {
+        const uint32_t ver = S.hosp_version;
+        if (!same_rect(current_area(), D.hosp[ver]) && wsa <= h) {
+          goto_area(D.hosp[ver]);
+          set_area(AREA_HOSP0 + ver);
+          wsa = h;
+        }
+        w &= ~F_WQ;
+      } else if (hd == 16) {
+        w |= F_WQ;
+      } else if (!(w & F_WQ) && can_move()) {
+        move_agent_from();
+      }
+    } else {  // NA
+      if (hd == 8) {
+        goto_area(D.housing);
+        set_area(AREA_HOUSING);
+      } else if (hd == 12) {
+        goto_area(home());
+        set_area(AREA_HOME);
+      } else {
+        move_agent_from();
+      }
+    }
+    update_infection_dynamics(tm_arr, gi);
+  }
+
+  // citizen/mod.rs:351-365 with allocation_map.rs:144-150
+  __device__ void hospitalize() {
+    if ((w & F_HOSP) || state() != ST_I || sev() != SEV_SEVERE) return;
+    if (!((D.hosp_window >> window_of(D, w)) & 1u)) return;
+    const uint32_t fv = S.first_vacant;
+    if (fv != ~0u) {
+      const Rect hr = D.hosp[S.hosp_version];
+      const uint32_t wd = static_cast<uint32_t>(hr.x1 - hr.x0 + 1);
+      tx = hr.x0 + static_cast<int32_t>(fv % wd);
+      ty = hr.y0 + static_cast<int32_t>(fv / wd);
+      w |= F_HOSP;
+    } else {
+      int32_t px, py;
+      random_point(home(), px, py);
+      move_to(px, py);
+    }
+  }
+
+  // citizen/mod.rs:397-413 + default_disease_handler.rs:88-102
+  __device__ void deceased() {
+    if (state() == ST_I) {
+      const uint32_t day = fld(w, B_DAY, M_DAY), sv = sev();
+      uint32_t ns = ST_I;
+      if (sv == SEV_ASYM && day == 9u) ns = ST_R;        // ASYMPTOMATIC_LAST_DAY
+      else if (sv == SEV_MILD && day == 12u) ns = ST_R;  // MILD_INFECTED_LAST_DAY
+      else if (sv == SEV_SEVERE && day == D.last_day) {
+        const u32x4& d = disease_words();
+        ns = bernoulli(D.thr_death, join64(d.w[2], d.w[3])) ? ST_D : ST_R;
+      }
+      if (ns != ST_I) {
+        w = set_fld(w, B_STATE, M_STATE, ns);
+        w = set_fld(w, B_SEV, M_SEV, 0);
+        w = set_fld(w, B_DAY, M_DAY, 0);
+        tm = 0;
+      }
+    }
+    if (state() == ST_R) {
+      int32_t px, py;
+      random_point(home(), px, py);
+      move_to(px, py);
+    }
+    if ((state() == ST_R || state() == ST_D) && (w & F_HOSP)) w &= ~F_HOSP;
+  }
+};
+
+__device__ __forceinline__ void add_event(const DevView& v, const SimDesc& D, SimState* S, uint32_t hour, uint32_t kind) {
+  const uint32_t n = S->n_events;
+  if (n < D.event_cap) v.events[D.event_off + n] = Event{hour, kind};
+  S->n_events = n + 1;
+}
+
+// Hour epilogue for one simulation (k_epilogue, after k_step):
+// the CSV row (counts.rs:126-140), interventions (allocation_map.rs:306-337,
+// lockdown.rs:55-73, hospital.rs:51-74, vaccination.rs:52-54) and the standalone stop
+// rule (epidemiology_simulation.rs:564-575).
+__device__ void hour_epilogue(const DevView& v, const SimDesc& D, SimState* S, uint32_t h, const uint32_t* c) {
+  v.rows[D.row_off + (h - 1)] = Counts7{h, c[0], c[1], c[2], c[3], c[4], c[5]};
+  const uint32_t infected = c[2];
+  uint32_t sweep = 0;
+  if (D.n_vacc) {
+    // schedule sorted by hour, one entry per hour (the last config entry wins)
+    uint32_t lo = 0, hi = D.n_vacc;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (v.vacc_hour[D.vacc_off + mid] < h) lo = mid + 1; else hi = mid;
+    }
+    if (lo < D.n_vacc && v.vacc_hour[D.vacc_off + lo] == h) {
+      sweep |= SWEEP_VACC;
+      S->vacc_thr = v.vacc_thr[D.vacc_off + lo];
+      S->vacc_hour = h;
+      add_event(v, D, S, h, EV_VACCINATION);
+    }
+  }
+  if (D.has_lockdown && !S->locked && h % 24u == 0 && infected > D.lockdown_at) {
+    S->locked = 1;
+    sweep |= SWEEP_LOCK;
+    add_event(v, D, S, h, EV_LOCKED);
+  }
+  if (S->locked && h == S->zero_infection_hour + 21u * 24u) {
+    sweep |= SWEEP_UNLOCK;
+    S->locked = 0;
+    S->zero_infection_hour = 0;
+    add_event(v, D, S, h, EV_UNLOCKED);
+  }
+  if (h % 24u == 0) S->hosp_new_infections = infected > S->hosp_new_infections ? infected - S->hosp_new_infections : 0u;
+  if (!S->hosp_applied && h % 24u == 0 && D.has_hospital && S->hosp_new_infections >= D.spread_rate_threshold) {
+    S->hosp_version = 1;
+    S->hosp_applied = 1;
+    add_event(v, D, S, h, EV_HOSPITAL);
+  }
+  S->sweep = sweep;
+  if ((c[1] == 0 && c[2] == 0 && c[3] == 0) || h + 1 >= D.hours) S->active = 0;
+  S->hour = h;
+  S->rows_written = h;
+  S->commit_pending = 1;
+  S->first_vacant = ~0u;  // this hour's scan is consumed; the next one starts fresh
+}
+
+__global__ void k_step(DevView v) {
+  const BlockRef bm = v.agent_blocks[blockIdx.x];
+  const uint32_t sim = first_lane(bm.sim);
+  const SimDesc& D = v.desc[sim];
+  SimState* S = v.state + sim;
+  if (!S->active) return;  // uniform over the block
+  const uint32_t h = S->hour + 1;
+  const uint32_t hd = h % 24u;
+  const uint32_t local = first_lane(bm.first) + threadIdx.x;
+  const bool valid = local < D.n;
+  const uint8_t* occ = v.occ + D.cell_off;
+
+  uint32_t cat = 6;
+  if (valid) {
+    const uint64_t gi = D.agent_off + local;
+    Agent a(D, *S, occ);
+    a.id = local;
+    a.h = h;
+    a.w = v.st[gi];
+    const uint32_t p = v.pos[gi];
+    a.x = static_cast<int32_t>(p & 0xFFFFu);
+    a.y = static_cast<int32_t>(p >> 16);
+    a.tx = a.x;
+    a.ty = a.y;
+    a.tm = 0;
+    const uint32_t w0 = a.w;
+    const bool staff = a.ws() == WS_STAFF;
+    a.wsa = staff ? v.wsa[gi] : 0u;
+    const uint32_t wsa0 = a.wsa;
+    bool tm_dirty = false;
+
+    if (hd == 0) {
+      if (a.state() == ST_I) {  // increment_infection_day (state.rs:69-73)
+        const uint32_t day = fld(a.w, B_DAY, M_DAY);
+        a.w = set_fld(a.w, B_DAY, M_DAY, day < kMaxDay ? day + 1 : day);
+      }
+      a.hospitalize();
+    } else if (hd <= 6) {
+      if (!staff) a.set_area(AREA_HOME);
+    } else if (hd == 23) {
+      const uint32_t s0 = a.state();
+      a.deceased();
+      tm_dirty = a.state() != s0;
+    } else {
+      const uint32_t s0 = a.state(), sv0 = a.sev();
+      a.perform_movements(hd, v.tm, gi);
+      tm_dirty = a.state() != s0 || a.sev() != sv0;
+    }
+    if (a.tx != a.x || a.ty != a.y) {
+      a.w |= F_MOVE;
+      v.tgt[gi] = static_cast<uint32_t>(a.tx) | (static_cast<uint32_t>(a.ty) << 16);
+      const uint64_t ci = D.cell_off + static_cast<uint64_t>(a.ty) * D.stride + static_cast<uint32_t>(a.tx);
+      atomicMin(&v.claim[ci], claim_key(h, local));
+    }
+    if (grid_byte(D, a.w) != grid_byte(D, w0)) a.w |= F_DIRTY;
+    if (a.w != w0) v.st[gi] = a.w;
+    if (tm_dirty) v.tm[gi] = a.tm;
+    if (staff && a.wsa != wsa0) v.wsa[gi] = a.wsa;
+    const uint32_t s = a.state();
+    cat = s == ST_S ? 0u : s == ST_E ? 1u : s == ST_I ? ((a.w & F_HOSP) ? 3u : 2u) : s == ST_R ? 4u : 5u;
+  }
+
+  // S/E/I/H/R/D via wavefront ballots into LDS, then one plain store per block and
+  // category; k_epilogue reduces the block partials.
+  __shared__ uint32_t s_counts[6];
+  if (threadIdx.x < 6) s_counts[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (uint32_t k = 0; k < 6; ++k) {
+    const unsigned long long m = __ballot(cat == k);
+    if (lane == 0 && m) atomicAdd(&s_counts[k], static_cast<uint32_t>(__popcll(m)));
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) v.block_counts[6u * blockIdx.x + threadIdx.x] = s_counts[threadIdx.x];
+}
+
+// One block per simulation: reduce the hour's block partial counts, then the epilogue.
+// Runs every hour; for a simulation that did not step this hour it only retires the
+// previous hour's commit.
+__global__ void k_epilogue(DevView v) {
+  const uint32_t sim = blockIdx.x;
+  const SimDesc& D = v.desc[sim];
+  SimState* S = v.state + sim;
+  if (!S->active) {
+    if (threadIdx.x == 0) S->commit_pending = 0;
+    return;
+  }
+  __shared__ uint32_t s_sum[6];
+  if (threadIdx.x < 6) s_sum[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t acc[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t b = threadIdx.x; b < D.n_blocks; b += blockDim.x) {
+    const uint32_t* bc = v.block_counts + 6u * (D.block_begin + b);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[k] += bc[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    uint32_t x = acc[k];
+    for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && x) atomicAdd(&s_sum[k], x);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) hour_epilogue(v, D, S, S->hour + 1, s_sum);
+}
+
+__global__ void k_commit(DevView v) {
+  const BlockRef bm = v.agent_blocks[blockIdx.x];
+  const uint32_t sim = first_lane(bm.sim);
+  const SimDesc& D = v.desc[sim];
+  SimState* S = v.state + sim;
+  if (!S->commit_pending) return;
+  const uint32_t h = S->hour;
+  const uint32_t sweep = S->sweep;
+  const uint32_t local = first_lane(bm.first) + threadIdx.x;
+  if (local < D.n) {
+    const uint64_t gi = D.agent_off + local;
+    const uint32_t w = v.st[gi];
+    uint32_t nw = w;
+    if (sweep) {
+      if ((sweep & SWEEP_VACC) && fld(w, B_STATE, M_STATE) == ST_S) {  // allocation_map.rs:381-387
+        const u32x4 d = philox4x32_10(local, S->vacc_hour, SITE_VACCINATE, 0, D.seed_lo, D.seed_hi);
+        if (bernoulli(S->vacc_thr, join64(d.w[0], d.w[1]))) nw |= F_VACC;
+      }
+      if ((sweep & SWEEP_LOCK) && fld(w, B_WS, M_WS) != WS_ESSENTIAL) nw |= F_ISO;  // :349-356
+      if (sweep & SWEEP_UNLOCK) nw &= ~F_ISO;                                          // :358-365
+    }
+    if (w & (F_MOVE | F_DIRTY)) {
+      uint8_t* occ = v.occ + D.cell_off;
+      const uint32_t p = v.pos[gi];
+      const uint64_t old_ci = static_cast<uint64_t>(p >> 16) * D.stride + (p & 0xFFFFu);
+      const uint8_t byte = grid_byte(D, w);
+      bool moved = false;
+      if (w & F_MOVE) {
+        const uint32_t t = v.tgt[gi];
+        const uint64_t ci = static_cast<uint64_t>(t >> 16) * D.stride + (t & 0xFFFFu);
+        if (v.claim[D.cell_off + ci] == claim_key(h, local)) {
+          occ[old_ci] = 0;
+          occ[ci] = byte;
+          v.pos[gi] = t;
+          moved = true;
+        }
+      }
+      if (!moved && (w & F_DIRTY)) occ[old_ci] = byte;
+      nw &= ~(F_MOVE | F_DIRTY);
+    }
+    if (nw != w) v.st[gi] = nw;
+  }
+}
+
+// First vacant cell of the current hospital area in row-major order (area.rs:119-146 +
+// allocation_map.rs:145), reduced with atomicMin over the snapshot grid.
+__global__ void k_hospital_scan(DevView v, uint32_t cells_per_thread) {
+  const BlockRef bm = v.hosp_blocks[blockIdx.x];
+  const uint32_t sim = first_lane(bm.sim);
+  const SimDesc& D = v.desc[sim];
+  SimState* S = v.state + sim;
+  if (!S->active || (S->hour + 1) % 24u != 0) return;
+  const uint32_t first = first_lane(bm.first);
+  if (__hip_atomic_load(&S->first_vacant, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < first)
+    return;  // an earlier block already found one
+  const Rect hr = D.hosp[S->hosp_version];
+  const uint32_t wd = static_cast<uint32_t>(hr.x1 - hr.x0 + 1);
+  const uint32_t ncell = wd * static_cast<uint32_t>(hr.y1 - hr.y0 + 1);
+  const uint8_t* occ = v.occ + D.cell_off;
+  uint32_t best = ~0u;
+  for (uint32_t k = 0; k < cells_per_thread; ++k) {
+    const uint32_t c = first + k * blockDim.x + threadIdx.x;
+    if (c >= ncell) break;
+    const uint32_t cx = static_cast<uint32_t>(hr.x0) + c % wd, cy = static_cast<uint32_t>(hr.y0) + c / wd;
+    if (occ[static_cast<uint64_t>(cy) * D.stride + cx] == 0) { best = c; break; }
+  }
+  // wavefront min then one atomic per wave
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(best, off, kWave);
+    best = o < best ? o : best;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0 && best != ~0u) atomicMin(&S->first_vacant, best);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long value) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t k = i; k < n; k += stride) p[k] = value;
+}
+
+}  // namespace
+
+hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStream_t stream) {
+  if (s.hosp_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hospital_scan, dim3(s.hosp_blocks), dim3(s.hosp_block), 0, stream, v, s.hosp_cells_per_thread);
+  return hipGetLastError();
+}
+
+hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream) {
+  if (s.agent_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_step, dim3(s.agent_blocks), dim3(s.block), 0, stream, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_epilogue(const DevView& v, const LaunchShape& s, hipStream_t stream) {
+  if (s.n_sims == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_epilogue, dim3(s.n_sims), dim3(256), 0, stream, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_commit(const DevView& v, const LaunchShape& s, hipStream_t stream) {
+  if (s.agent_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_commit, dim3(s.agent_blocks), dim3(s.block), 0, stream, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long value, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_fill_u64, dim3(static_cast<uint32_t>(blocks < 65536 ? blocks : 65536)), dim3(256), 0, stream, p,
+                     n, value);
+  return hipGetLastError();
+}
+
+}  // namespace epi

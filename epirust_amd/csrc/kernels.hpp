@@ -1,0 +1,46 @@
+// Launch interface of the hour-step kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+#include "model.hpp"
+
+namespace epi {
+
+// Device pointers handed to every kernel by value.
+struct DevView {
+  uint32_t* pos;
+  uint32_t* st;
+  uint32_t* tm;
+  uint32_t* wsa;
+  uint32_t* tgt;
+  uint8_t* occ;
+  unsigned long long* claim;
+  const SimDesc* desc;
+  SimState* state;
+  const BlockRef* agent_blocks;  // per agent block: (sim, first local agent)
+  const BlockRef* hosp_blocks;   // per hospital-scan block: (sim, first cell)
+  Counts7* rows;
+  Event* events;
+  const uint32_t* vacc_hour;
+  const unsigned long long* vacc_thr;
+  uint32_t* block_counts;  // 6 per agent block: the hour's partial S E I H R D
+};
+
+struct LaunchShape {
+  uint32_t block = 256;          // threads per agent block
+  uint32_t agent_blocks = 0;     // total agent blocks over the batch
+  uint32_t hosp_block = 256;     // threads per hospital-scan block
+  uint32_t hosp_cells_per_thread = 8;
+  uint32_t hosp_blocks = 0;
+  uint32_t n_sims = 0;
+};
+
+hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStream_t stream);
+hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream);
+hipError_t launch_epilogue(const DevView& v, const LaunchShape& s, hipStream_t stream);
+hipError_t launch_commit(const DevView& v, const LaunchShape& s, hipStream_t stream);
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long value, hipStream_t stream);
+
+}  // namespace epi

@@ -1,0 +1,124 @@
+// Agent state layout in HBM and the per-simulation descriptors shared by the host
+// runtime and the gfx950 kernels.
+//
+// Agent state is struct-of-arrays, one 32-bit word per field so a wavefront's 64
+// lanes read 256 contiguous bytes per field:
+//   pos[i]  x | y << 16                       (grid_size <= 32000)
+//   st[i]   packed state word (bit layout below)
+//   tm[i]   Exposed{at_hour} / Pre{at_hour}
+//   wsa[i]  HospitalStaff{work_start_at}      (read only for the 0.2% staff)
+//   tgt[i]  the cell chosen this hour          (step -> commit hand-off)
+// The grid is a dense (stride x (g+1)) byte array: 0 vacant, 1 occupied by an agent
+// that cannot infect, 2 / 3 occupied by an infectious agent in the regular / high
+// transmission window. Claims are a parallel u64 array keyed by (hour, agent id).
+#pragma once
+#include <cstdint>
+
+namespace epi {
+
+// ---- state word
+enum : uint32_t {
+  ST_S = 0, ST_E = 1, ST_I = 2, ST_R = 3, ST_D = 4,
+  SEV_PRE = 0, SEV_ASYM = 1, SEV_MILD = 2, SEV_SEVERE = 3,
+  WS_NORMAL = 0, WS_ESSENTIAL = 1, WS_STAFF = 2, WS_NA = 3,
+  AREA_HOME = 0, AREA_WORK = 1, AREA_TRANSPORT = 2, AREA_HOUSING = 3, AREA_HOSP0 = 4, AREA_HOSP1 = 5,
+};
+
+constexpr uint32_t B_STATE = 0, M_STATE = 0x7u;
+constexpr uint32_t B_SEV = 3, M_SEV = 0x3u;
+constexpr uint32_t F_HOSP = 1u << 5;
+constexpr uint32_t F_VACC = 1u << 6;
+constexpr uint32_t F_ISO = 1u << 7;
+constexpr uint32_t F_WQ = 1u << 8;
+constexpr uint32_t B_WS = 9, M_WS = 0x3u;
+constexpr uint32_t B_AREA = 11, M_AREA = 0x7u;
+constexpr uint32_t F_PT = 1u << 14;
+constexpr uint32_t F_MOVE = 1u << 15;   // transient: agent targets another cell this hour
+constexpr uint32_t F_DIRTY = 1u << 16;  // transient: the agent's grid byte changes this hour
+constexpr uint32_t B_IMM = 17, M_IMM = 0x7u;  // immunity + 2
+constexpr uint32_t B_DAY = 20, M_DAY = 0xFFFu;
+constexpr uint32_t kMaxDay = M_DAY;
+
+// ---- Philox sites (counter word 2); the hour word is 0xFFFFFFFF during population synthesis
+constexpr uint32_t kInitHour = 0xFFFFFFFFu;
+enum : uint32_t {
+  SITE_MOVE = 1,       // w0 x, w1 y of the hour's one random area point; w2 neighbour choice
+  SITE_DISEASE = 2,    // w0 exposed random factor; (w2,w3) the hour's disease Bernoulli
+  SITE_INFECT = 3,     // blocks 0..3: Moore neighbour k uses block k>>1, words 2(k&1), 2(k&1)+1
+  SITE_VACCINATE = 4,  // (w0,w1)
+  SITE_INIT_A = 16,    // (w0,w1) working, (w2,w3) public transport
+  SITE_INIT_B = 17,    // (w0,w1) hospital staff, w2 immunity
+  SITE_INIT_C = 18,    // (w0,w1) essential worker, (w2,w3) starting-infection key
+  SITE_INIT_HOUSE = 19 // counter id = house index; w0,w1 x permutation, w2,w3 y permutation
+};
+
+// ---- sweeps decided after an hour's counts, applied by the commit kernel
+enum : uint32_t { SWEEP_VACC = 1, SWEEP_LOCK = 2, SWEEP_UNLOCK = 4 };
+// ---- intervention event kinds (intervention_reporter.rs names)
+enum : uint32_t { EV_VACCINATION = 0, EV_LOCKED = 1, EV_UNLOCKED = 2, EV_HOSPITAL = 3 };
+
+struct Rect {
+  int32_t x0, y0, x1, y1;
+};
+
+// Read-only per-simulation descriptor (device copy built once at create).
+struct SimDesc {
+  uint32_t n;          // agents
+  uint32_t g;          // grid_size
+  uint32_t stride;     // grid row pitch (>= g+1)
+  uint32_t hours;      // config hours: the loop runs 1..hours-1
+  uint64_t agent_off;  // first agent in the concatenated SoA
+  uint64_t cell_off;   // first cell in the concatenated grid / claim arrays
+  uint64_t row_off;    // first row in the concatenated per-hour counts
+  uint64_t event_off;  // first slot in the concatenated event log
+  uint32_t event_cap;
+  uint32_t seed_lo, seed_hi;
+  Rect housing, transport, work, hosp[2];
+  int32_t house_sx, house_sy, house_nx;
+  uint32_t n_houses;
+  int32_t office_sx, office_sy, office_nx;
+  uint32_t n_offices;
+  // disease (common/src/disease/mod.rs:26-44)
+  uint32_t regular_start, high_start, last_day;
+  uint32_t exposed_duration, pre_symptomatic_duration;
+  uint64_t thr_regular, thr_high, thr_death, thr_symptoms, thr_severe;
+  uint32_t hosp_window;  // bit w: is_to_be_hospitalized holds in transmission window w
+  // interventions
+  uint32_t has_lockdown, lockdown_at;
+  uint32_t has_hospital, spread_rate_threshold;
+  uint32_t vacc_off, n_vacc;  // sorted, last-entry-wins schedule in the shared vacc arrays
+  uint32_t n_blocks;          // agent blocks of this simulation (step / commit)
+  uint32_t block_begin;       // index of its first agent block
+};
+
+// Mutable per-simulation scalars, owned by the device.
+struct SimState {
+  uint32_t hour;            // last hour stepped
+  uint32_t active;          // 0 once stopped (E=I=H=0) or out of hours
+  uint32_t commit_pending;  // step ran, commit not yet
+  uint32_t sweep;           // SWEEP_* for the commit of `hour`
+  uint32_t first_vacant;    // row-major index in the hospital, or ~0u
+  uint32_t hosp_version;    // 0 after resize_hospital, 1 after increase_hospital_size
+  uint32_t locked, zero_infection_hour;
+  uint32_t hosp_new_infections, hosp_applied;
+  uint32_t vacc_hour;
+  uint32_t n_events;
+  uint64_t vacc_thr;
+  uint32_t rows_written;
+  uint32_t error;
+};
+
+// Launch-table entry: (simulation, first local agent) or (simulation, first hospital cell).
+struct BlockRef {
+  uint32_t sim, first;
+};
+
+struct Counts7 {
+  uint32_t hour, s, e, i, h, r, d;
+};
+
+struct Event {
+  uint32_t hour, kind;
+};
+
+}  // namespace epi

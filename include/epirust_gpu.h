@@ -1,0 +1,131 @@
+/*
+ * epirust_gpu.h -- C ABI of the MI355X (gfx950) per-hour agent step for one EpiRust
+ * engine region.
+ *
+ * The library owns all device memory; callers own every host buffer they pass. A handle
+ * is driven by one caller thread at a time (calls on one handle are serialized on its
+ * own HIP stream); separate handles may be used concurrently. No entry point aborts:
+ * conditions on which the reference engine panics return a nonzero status and
+ * epi_last_error() returns the reference's panic text.
+ *
+ * Reference paths below are relative to the reference repository root.
+ */
+#ifndef EPIRUST_GPU_H
+#define EPIRUST_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define EPI_OK 0
+#define EPI_ERR_CONFIG 1   /* config JSON rejected (serde would fail)                 */
+#define EPI_ERR_PANIC 2    /* a condition on which the reference engine panics        */
+#define EPI_ERR_DEVICE 3   /* HIP runtime failure or no gfx950 device                 */
+#define EPI_ERR_ARG 4      /* invalid argument (null pointer, index out of range)     */
+#define EPI_ERR_IO 5       /* output file could not be written                        */
+
+/* flags for epi_create / epi_batch_create */
+#define EPI_FLAG_DETERMINISTIC 1u   /* Philox streams keyed (seed, agent id, hour); the only mode */
+#define EPI_FLAG_NO_GRAPH 2u        /* launch hour kernels directly instead of replaying day graphs */
+
+/* One CSV row of the epicurve; mirrors Counts (engine/src/models/events/counts.rs:25-34). */
+typedef struct {
+  uint32_t hour, susceptible, exposed, infected, hospitalized, recovered, deceased;
+} epi_counts;
+
+/* One entry of the interventions report (engine/src/listeners/intervention_reporter.rs:28-63).
+ * kind: 0 vaccination, 1 lockdown {"status": "locked_down"}, 2 lockdown {"status":
+ * "lockdown_revoked"}, 3 build_new_hospital. */
+typedef struct {
+  uint32_t hour, kind;
+} epi_event;
+
+/* Per-agent dump for parity checks; caller allocates n_agents entries per array
+ * (current_area: 4 per agent, x0 y0 x1 y1). Field meaning follows Citizen
+ * (engine/src/citizen/mod.rs:48-63) and State (engine/src/state_machine/state.rs:22-37). */
+typedef struct {
+  int32_t* x;
+  int32_t* y;
+  uint8_t* state;         /* 0 Susceptible, 1 Exposed, 2 Infected, 3 Recovered, 4 Deceased */
+  uint8_t* severity;      /* Infected only: 0 Pre, 1 Asymptomatic, 2 Mild, 3 Severe */
+  uint32_t* infection_day;
+  uint32_t* at_hour;      /* Exposed{at_hour} / Pre{at_hour}, else 0 */
+  uint8_t* flags;         /* 1 hospitalized, 2 vaccinated, 4 isolated, 8 work_quarantined, 16 uses_public_transport */
+  uint8_t* work_status;   /* 0 Normal, 1 Essential, 2 HospitalStaff, 3 NA */
+  uint32_t* work_start_at;
+  int8_t* immunity;
+  int32_t* current_area;
+} epi_agents_soa;
+
+typedef struct epi_sim epi_sim;
+
+/* Replaces Epidemiology::new (engine/src/epidemiology_simulation.rs:75-135), called from
+ * EngineApp::start_standalone (engine/src/engine_app.rs:89-125) with the config read by
+ * Config::read (common/src/config/mod.rs:119-123). config_json is the reference's
+ * standalone config schema; `seed` keys the Philox streams. */
+int epi_create(const char* config_json, uint64_t seed, uint32_t flags, epi_sim** out);
+
+/* Batched replicates: n independent simulations sharing every launch. Same role as the
+ * reference's serial `launch-batch` of engine-app runs (engine/scripts/epirust:58-99). */
+int epi_batch_create(const char* const* configs, const uint64_t* seeds, uint32_t n, uint32_t flags, epi_sim** out);
+
+/* Runs up to n_hours of the hour loop (epidemiology_simulation.rs:223-258): the step
+ * (CitizenLocationMap::simulate, engine/src/allocation_map.rs:67-129), the counts row,
+ * process_interventions (allocation_map.rs:306-337) and the standalone stop rule
+ * (epidemiology_simulation.rs:564-575). rows receives one epi_counts per hour run. */
+int epi_step(epi_sim* sim, uint32_t n_hours, epi_counts* rows, uint32_t* n_written, int* stopped);
+
+/* Batch form: rows is n x n_hours (row-major by simulation); n_written and stopped have
+ * one entry per simulation. */
+int epi_batch_step(epi_sim* sim, uint32_t n_hours, epi_counts* rows, uint32_t* n_written, int* stopped);
+
+/* The whole run plus its output files: replaces Epidemiology::run / run_single_engine
+ * (epidemiology_simulation.rs:194-274) with the CSV listener (listeners/csv_service.rs:44-71)
+ * and interventions report (listeners/intervention_reporter.rs:28-63), written to
+ * <output_dir>/output/simulation_0_<UTC time>.csv and ..._interventions.json
+ * (engine/src/utils/util.rs:28-43). final_path (optional, cap bytes) gets the CSV path. */
+int epi_run(epi_sim* sim, const char* output_dir, char* final_path, size_t cap);
+
+/* Current per-agent state of simulation `index` of the handle (0 for epi_create). */
+int epi_read_agents(epi_sim* sim, uint32_t index, epi_agents_soa* dst);
+
+/* Counts of the last completed hour (hour 0: counts_at_start, engine/src/utils/util.rs:45-51). */
+int epi_current_counts(epi_sim* sim, uint32_t index, epi_counts* out);
+
+/* Intervention events recorded so far for simulation `index`. */
+int epi_events(epi_sim* sim, uint32_t index, epi_event* out, uint32_t cap, uint32_t* n);
+
+/* Sizes of a handle: simulations, and agents of simulation `index`. */
+int epi_info(epi_sim* sim, uint32_t index, uint32_t* n_sims, uint32_t* n_agents, uint32_t* grid_size, uint32_t* hours);
+
+/* Geometry of simulation `index`: housing, transport, work, initial hospital and the
+ * hospital after BuildNewHospital (5 x {x0,y0,x1,y1}) and the current hospital version. */
+int epi_geometry(epi_sim* sim, uint32_t index, int32_t rects[20], uint32_t* hospital_version);
+
+/* Profiling: runs n_hours with a HIP event pair around every kernel on the handle's stream
+ * and returns the mean duration in ms of each kernel (hospital scan, step, commit; the
+ * hospital scan mean is over the hours that ran it) and how many hours ran. */
+int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t launches[3], uint32_t* hours_run);
+
+/* Launch shape: threads per agent block (64..1024, multiple of 64). Takes effect on the
+ * next step; discards the captured day graph. */
+int epi_set_block_size(epi_sim* sim, uint32_t threads);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* epi_last_error(void);
+
+void epi_destroy(epi_sim* sim);
+
+/* Command-line entry equivalent to engine-app (engine-app/src/main.rs:60-87, 167-173):
+ * -c/--config, -m/--mode (standalone only), -i/--id, -t/--threads (accepted, unused),
+ * -o/--output-dir (default /tmp). Returns a process exit code. */
+int epi_cli_main(int argc, char** argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

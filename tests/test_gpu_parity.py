@@ -1,0 +1,170 @@
+"""Bit-exact parity of the HIP step (through the C ABI) with the CPU oracle, in
+deterministic mode: every agent's state, position and counters, hour by hour."""
+import copy
+import json
+import os
+
+import numpy as np
+import pytest
+
+import epirust_amd as G
+from oracle import epicpu as O
+from tests.gpu_helpers import assert_agents_equal, interventions_config, say
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def run_lockstep(cfg, hours, label, every=1, block=None, seed=None):
+    say("[parity] %s: lockstep %d hours" % (label, hours))
+    gpu = G.Simulation(cfg, seed=seed)
+    if block:
+        gpu.set_block_size(block)
+    cpu = O.OracleSim(cfg, seed=seed)
+    assert_agents_equal(gpu.agents(), cpu.agents(), "%s hour 0" % label)
+    h = 0
+    while h < hours:
+        k = min(every, hours - h)
+        g_rows, g_stop = gpu.step(k)
+        c_rows, c_stop = cpu.step(k)
+        assert np.array_equal(g_rows, c_rows), "%s rows differ after hour %d" % (label, h)
+        assert g_stop == c_stop
+        h += len(c_rows)
+        assert_agents_equal(gpu.agents(), cpu.agents(), "%s hour %d" % (label, h))
+        if c_stop:
+            break
+    assert gpu.events() == cpu.events()
+    gpu.close()
+    cpu.close()
+    return h
+
+
+def test_initial_population_matches(cfg1):
+    say("[parity] initial population cfg1")
+    gpu = G.Simulation(cfg1)
+    cpu = O.OracleSim(cfg1)
+    assert_agents_equal(gpu.agents(), cpu.agents(), "init")
+    assert gpu.counts() == cpu.counts()
+    (rects, ver) = gpu.geometry()
+    c_rects, _ = cpu.geometry()
+    assert rects[:4] == c_rects[:4] and ver == 0
+
+
+def test_cfg1_every_hour_bit_exact(cfg1):
+    """10,000 agents on 250x250, 2,399 hours: all agents compared after every hour."""
+    h = run_lockstep(cfg1, 2399, "cfg1")
+    assert h > 0
+
+
+def test_interventions_every_hour_bit_exact(cfg1):
+    cfg = interventions_config(cfg1)
+    run_lockstep(cfg, 900, "interventions")
+
+
+def test_hospital_growth_and_full_hospital(cfg1):
+    """Tiny hospital (beds 0) fills up, so admissions fall back to a home cell
+    (allocation_map.rs:148); a BuildNewHospital then grows it to (x0,0)-(g,g)."""
+    cfg = interventions_config(cfg1, n=6000, g=200)
+    cfg["geography_parameters"]["hospital_beds_percentage"] = 0.0
+    cfg["disease"]["percentage_severe_infected_population"] = 0.9
+    cfg["interventions"] = [{"BuildNewHospital": {"spread_rate_threshold": 60}}]
+    run_lockstep(cfg, 1500, "hospital", every=3)
+
+
+def test_zero_transmission_closed_forms_through_gpu(cfg1):
+    for asym, sev, death in ((1.0, 0.0, 0.0), (0.0, 1.0, 1.0), (0.0, 1.0, 0.0)):
+        cfg = copy.deepcopy(cfg1)
+        d = cfg["disease"]
+        d.update(regular_transmission_rate=0.0, high_transmission_rate=0.0, percentage_asymptomatic_population=asym,
+                 percentage_severe_infected_population=sev, death_rate=death)
+        cfg["starting_infections"]["exposed"] = 5
+        run_lockstep(cfg, 700, "zero-transmission %s" % ((asym, sev, death),), every=7)
+
+
+@pytest.mark.parametrize("block", [64, 128, 512, 1024])
+def test_block_shapes_do_not_change_results(cfg1, block):
+    cfg = interventions_config(cfg1, n=3000, g=140)
+    run_lockstep(cfg, 300, "block%d" % block, every=5, block=block)
+
+
+def test_whole_run_with_day_graphs_matches(cfg1):
+    """One epi_step call covering the run: whole days replay the captured hipGraph."""
+    say("[parity] cfg1 graph-replayed run")
+    cfg = copy.deepcopy(cfg1)
+    cfg["disease"]["regular_transmission_rate"] = 0.1
+    cfg["disease"]["high_transmission_rate"] = 0.4
+    cfg["interventions"] = [{"Lockdown": {"at_number_of_infections": 50, "essential_workers_population": 0.1}},
+                            {"Vaccinate": {"at_hour": 480, "percent": 0.2}}]
+    gpu = G.Simulation(cfg)
+    cpu = O.OracleSim(cfg)
+    g_rows, g_stop = gpu.step(2399)
+    c_rows, c_stop = cpu.step(2399)
+    assert g_stop == c_stop
+    assert np.array_equal(g_rows, c_rows)
+    assert_agents_equal(gpu.agents(), cpu.agents(), "after whole run")
+    assert gpu.events() == cpu.events()
+
+
+def test_seeds_change_the_run(cfg1):
+    say("[parity] seeds")
+    a = G.Simulation(cfg1, seed=1)
+    b = G.Simulation(cfg1, seed=2)
+    ra, _ = a.step(300)
+    rb, _ = b.step(300)
+    assert not np.array_equal(a.agents()["x"], b.agents()["x"])
+    c = O.OracleSim(cfg1, seed=2)
+    rc, _ = c.step(300)
+    assert np.array_equal(rb, rc)
+
+
+def test_batch_of_mixed_replicates_matches_single_runs(cfg1):
+    """Replicates of different sizes and parameters in one launch: each equals its own run."""
+    rng = np.random.default_rng(7)
+    cfgs, seeds = [], []
+    for r in range(6):
+        c = copy.deepcopy(cfg1)
+        n = int(rng.integers(2000, 9000))
+        c["population"]["Auto"]["number_of_agents"] = n
+        c["geography_parameters"]["grid_size"] = int(round((n / 0.25) ** 0.5))
+        c["disease"]["regular_transmission_rate"] = float(rng.uniform(0.05, 0.2))
+        c["disease"]["high_transmission_rate"] = float(rng.uniform(0.2, 0.5))
+        c["disease"]["death_rate"] = float(rng.uniform(0.05, 0.3))
+        c["hours"] = int(rng.integers(300, 700))
+        c["starting_infections"]["exposed"] = 5
+        if r % 2:
+            c["interventions"] = [{"Lockdown": {"at_number_of_infections": 30, "essential_workers_population": 0.1}}]
+        cfgs.append(c)
+        seeds.append(1000 + r)
+    say("[parity] batch of %d" % len(cfgs))
+    batch = G.Batch(cfgs, seeds)
+    rows, stopped = batch.step(800)
+    for i, c in enumerate(cfgs):
+        cpu = O.OracleSim(c, seed=seeds[i])
+        c_rows, c_stop = cpu.step(800)
+        assert np.array_equal(rows[i], c_rows), "replicate %d rows" % i
+        assert stopped[i] == c_stop
+        assert_agents_equal(batch.agents(i), cpu.agents(), "replicate %d" % i)
+        assert batch.events(i) == cpu.events()
+
+
+def test_cfg2_1m_first_two_days_bit_exact():
+    """1,000,000 agents on 2000x2000: the first 48 hours compared agent by agent."""
+    cfg = json.load(open(os.path.join(HERE, "golden", "configs", "cfg2_1m.json")))
+    run_lockstep(cfg, 48, "cfg2", every=12)
+
+
+def test_cfg2_full_run_rows_match_oracle_fixture():
+    """Full 2,399-hour run at 1M agents: the epicurve rows and a checksum of the final
+    agent state equal the oracle's, stored as a fixture (tests/golden/make_golden.py)."""
+    path = os.path.join(HERE, "golden", "cfg2_1m_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated")
+    gold = np.load(path)
+    cfg = json.load(open(os.path.join(HERE, "golden", "configs", "cfg2_1m.json")))
+    say("[parity] cfg2 full run vs fixture")
+    gpu = G.Simulation(cfg)
+    rows, stopped = gpu.step(2399)
+    assert np.array_equal(rows, gold["rows"])
+    a = gpu.agents()
+    from tests.golden.make_golden import state_digest
+    assert state_digest(a) == str(gold["digest"])
