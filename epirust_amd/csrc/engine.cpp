@@ -352,6 +352,8 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     D.n = c.number_of_agents;
     D.g = G.g;
     D.stride = G.stride;
+    D.pitch = padded_pitch(G.stride);
+    D.height = G.height;
     D.hours = c.hours;
     D.agent_off = agent_off;
     D.cell_off = cell_off;
@@ -406,7 +408,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
         return fail(EPI_ERR_CONFIG, "grid_size too small for the engine's area layout");
     if (c.number_of_agents == 0 && c.hours > 1) s->any_empty = true;
     agent_off += D.n;
-    cell_off += static_cast<uint64_t>(G.stride) * G.height;
+    cell_off += padded_cells(G.stride, G.height);
     row_off += c.hours > 1 ? c.hours - 1 : 0;
     event_off += D.event_cap;
     s->max_hour = std::max(s->max_hour, c.hours > 1 ? c.hours - 1 : 0u);
@@ -471,11 +473,12 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
       HIP_TRY(hipMemcpy(v.tm + D.agent_off, P.tm.data(), nb, hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(v.wsa + D.agent_off, P.wsa.data(), nb, hipMemcpyHostToDevice));
     }
-    const size_t cells = static_cast<size_t>(P.geo.stride) * P.geo.height;
+    const size_t cells = padded_cells(P.geo.stride, P.geo.height);
     grid.assign(cells, 0);
     for (uint32_t a = 0; a < D.n; ++a) {
       const uint32_t p = P.pos[a];
-      grid[static_cast<size_t>(p >> 16) * P.geo.stride + (p & 0xFFFFu)] = host_grid_byte(s->cfg[i], P.st[a]);
+      grid[static_cast<size_t>((p >> 16) + kPadY) * D.pitch + (p & 0xFFFFu) + kPadX] =
+          host_grid_byte(s->cfg[i], P.st[a]);
     }
     HIP_TRY(hipMemcpy(v.occ + D.cell_off, grid.data(), cells, hipMemcpyHostToDevice));
     SimState st;

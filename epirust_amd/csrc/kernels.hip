@@ -38,8 +38,22 @@ __device__ __forceinline__ Rect building(int32_t sx, int32_t sy, int32_t nx, int
   return Rect{x0, y0, x0 + size - 1, y0 + size - 1};
 }
 
-__constant__ int8_t kMooreDx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};  // point.rs:57-61
-__constant__ int8_t kMooreDy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+// Moore neighbour order (point.rs:57-61): (-1,-1) (0,-1) (1,-1) (-1,0) (1,0) (-1,1) (0,1) (1,1)
+__device__ constexpr int moore_dx(int k) { return k < 3 ? k - 1 : (k == 3 ? -1 : (k == 4 ? 1 : k - 6)); }
+__device__ constexpr int moore_dy(int k) { return k < 3 ? -1 : (k < 5 ? 0 : 1); }
+
+// Compile-time loop over the eight Moore neighbours: f(Idx<k>) for k = 0..7.
+template <int V>
+struct Idx {
+  static constexpr int value = V;
+};
+template <int K = 0, class F>
+__device__ __forceinline__ void for_moore(F&& f) {
+  if constexpr (K < 8) {
+    f(Idx<K>{});
+    for_moore<K + 1>(f);
+  }
+}
 
 __device__ __forceinline__ uint32_t fld(uint32_t w, uint32_t b, uint32_t m) { return (w >> b) & m; }
 __device__ __forceinline__ uint32_t set_fld(uint32_t w, uint32_t b, uint32_t m, uint32_t v) {
@@ -62,6 +76,11 @@ __device__ __forceinline__ uint8_t grid_byte(const SimDesc& D, uint32_t w) {
   return static_cast<uint8_t>(1u + (infectious ? window_of(D, w) : 0u));
 }
 
+// Index of cell (x, y) in the padded grid / claim arrays of one simulation.
+__device__ __forceinline__ uint64_t cell_index(const SimDesc& D, int32_t x, int32_t y) {
+  return static_cast<uint64_t>(static_cast<uint32_t>(y) + kPadY) * D.pitch + static_cast<uint32_t>(x) + kPadX;
+}
+
 __device__ __forceinline__ unsigned long long claim_key(uint32_t hour, uint32_t id) {
   return (static_cast<unsigned long long>(0xFFFFFFFFu - hour) << 32) | id;
 }
@@ -77,6 +96,11 @@ struct Agent {
   bool have_mv = false, have_dz = false;
   int inf_block = -1;
   u32x4 mv, dz, inf;
+  // 5x5 grid window around (wx, wy), one 64-bit row per grid row: bytes of columns
+  // c0 .. c0+7 where c0 = (x + kPadX - 2) & ~3 and woff = x + kPadX - c0 (2..5).
+  bool have_win = false;
+  int32_t wx = 0, wy = 0, woff = 0;
+  uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;  // named, so no variable-index access exists
 
   __device__ Agent(const SimDesc& d, const SimState& s, const uint8_t* o) : D(d), S(s), occ(o) {}
 
@@ -127,10 +151,51 @@ struct Agent {
   __device__ __forceinline__ void set_area(uint32_t tag) { w = set_fld(w, B_AREA, M_AREA, tag); }
 
   __device__ __forceinline__ bool vacant(int32_t cx, int32_t cy) const {
-    return occ[static_cast<uint32_t>(cy) * D.stride + static_cast<uint32_t>(cx)] == 0;
+    return occ[cell_index(D, cx, cy)] == 0;
   }
   __device__ __forceinline__ bool in_grid(int32_t cx, int32_t cy) const {
     return cx >= 0 && cy >= 0 && cx < static_cast<int32_t>(D.g) && cy < static_cast<int32_t>(D.g);
+  }
+  // Ten independent aligned dword loads: rows cy-2..cy+2, 8 bytes each.
+  __device__ __forceinline__ void load_window(int32_t cx, int32_t cy) {
+    const uint32_t px = static_cast<uint32_t>(cx) + kPadX;
+    const uint32_t c0 = (px - 2u) & ~3u;
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(
+        occ + static_cast<uint64_t>(static_cast<uint32_t>(cy) + kPadY - 2u) * D.pitch + c0);
+    const uint32_t q = D.pitch >> 2;
+    uint32_t lo[5], hi[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+      lo[r] = p[r * q];
+      hi[r] = p[r * q + 1];
+    }
+    r0 = join64(hi[0], lo[0]);
+    r1 = join64(hi[1], lo[1]);
+    r2 = join64(hi[2], lo[2]);
+    r3 = join64(hi[3], lo[3]);
+    r4 = join64(hi[4], lo[4]);
+    wx = cx;
+    wy = cy;
+    woff = static_cast<int32_t>(px - c0);
+    have_win = true;
+  }
+  template <int R>
+  __device__ __forceinline__ uint64_t row() const {
+    static_assert(R >= 0 && R < 5, "window row");
+    if constexpr (R == 0) return r0;
+    else if constexpr (R == 1) return r1;
+    else if constexpr (R == 2) return r2;
+    else if constexpr (R == 3) return r3;
+    else return r4;
+  }
+  // Byte of cell (wx + dx, wy + DY), |dx| <= 2, |DY| <= 2.
+  template <int DY>
+  __device__ __forceinline__ uint32_t wbyte(int32_t dx) const {
+    return static_cast<uint32_t>(row<DY + 2>() >> (8 * (woff + dx))) & 0xFFu;
+  }
+  // Make the window cover the 3x3 neighbourhood of (cx, cy).
+  __device__ __forceinline__ void cover(int32_t cx, int32_t cy) {
+    if (!have_win || cx - wx > 1 || wx - cx > 1 || cy - wy > 1 || wy - cy > 1) load_window(cx, cy);
   }
   __device__ __forceinline__ void random_point(const Rect& a, int32_t& px, int32_t& py) {  // area.rs:76-81
     const u32x4& m = move_words();
@@ -143,28 +208,31 @@ struct Agent {
   }
 
   // citizen/mod.rs:415-432
-  __device__ void move_agent_from() {
+  __device__ __forceinline__ void move_agent_from() {
     if (!can_move()) return;
     const Rect a = current_area();
     int32_t lx = x, ly = y;
     if (!contains(a, x, y)) random_point(a, lx, ly);  // teleport origin, vacancy not checked
+    load_window(lx, ly);
     uint32_t cand = 0, n = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int32_t cx = lx + kMooreDx[k], cy = ly + kMooreDy[k];
-      if (contains(a, cx, cy) && in_grid(cx, cy) && vacant(cx, cy)) { cand |= 1u << k; ++n; }
-    }
+    for_moore([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      const int32_t cx = lx + moore_dx(k), cy = ly + moore_dy(k);
+      if (contains(a, cx, cy) && in_grid(cx, cy) && wbyte<moore_dy(k)>(moore_dx(k)) == 0) {
+        cand |= 1u << k;
+        ++n;
+      }
+    });
     if (n == 0) return;
     uint32_t pick = below(move_words().w[2], n);
-    // index of the pick-th set bit of cand
-    for (uint32_t j = 0; j < pick; ++j) cand &= cand - 1;
+    for (uint32_t j = 0; j < pick; ++j) cand &= cand - 1;  // index of the pick-th set bit
     const int k = __builtin_ctz(cand);
-    tx = lx + kMooreDx[k];
-    ty = ly + kMooreDy[k];
+    tx = lx + moore_dx(k);
+    ty = ly + moore_dy(k);
   }
 
   // citizen/mod.rs:367-395
-  __device__ void goto_area(const Rect& target) {
+  __device__ __forceinline__ void goto_area(const Rect& target) {
     bool override_movement = false;
     const uint32_t s = ws();
     if (s == WS_NORMAL || s == WS_ESSENTIAL) {
@@ -175,30 +243,42 @@ struct Agent {
     if (s != WS_NA) {
       int32_t px, py;
       random_point(target, px, py);
-      move_to(px, py);
+      load_window(px, py);  // vacancy of the point + the infection scan around it
+      if (wbyte<0>(0) == 0) { tx = px; ty = py; }
       return;
     }
     move_agent_from();
   }
 
   // default_disease_handler.rs:64-86 (OR over fixed per-neighbour draws)
-  __device__ bool on_susceptible() {
+  // The 3x3 scan around (tx, ty) = (wx + ox, wy + OY) from window rows OY+1..OY+3.
+  template <int OY>
+  __device__ __forceinline__ bool scan_infectious(const Rect& a, int32_t ox) {
+    bool hit = false;
+    for_moore([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if (hit) return;
+      const int32_t cx = tx + moore_dx(k), cy = ty + moore_dy(k);
+      if (!contains(a, cx, cy) || !in_grid(cx, cy)) return;
+      const uint32_t b = wbyte<OY + moore_dy(k)>(ox + moore_dx(k));
+      if (b < 2) return;
+      const uint64_t thr = b == 2 ? D.thr_regular : D.thr_high;
+      hit = bernoulli(thr, infect_word(k));
+    });
+    return hit;
+  }
+  __device__ __forceinline__ bool on_susceptible() {
     if ((w & F_WQ) || (w & F_VACC)) return false;
     const Rect a = current_area();
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int32_t cx = tx + kMooreDx[k], cy = ty + kMooreDy[k];
-      if (!contains(a, cx, cy) || !in_grid(cx, cy)) continue;
-      const uint8_t b = occ[static_cast<uint32_t>(cy) * D.stride + static_cast<uint32_t>(cx)];
-      if (b < 2) continue;
-      const uint64_t thr = b == 2 ? D.thr_regular : D.thr_high;
-      if (bernoulli(thr, infect_word(k))) return true;
-    }
-    return false;
+    cover(tx, ty);
+    const int32_t ox = tx - wx, oy = ty - wy;
+    if (oy < 0) return scan_infectious<-1>(a, ox);
+    if (oy > 0) return scan_infectious<1>(a, ox);
+    return scan_infectious<0>(a, ox);
   }
 
   // disease_state_machine.rs:53-70 + default_disease_handler.rs:41-62
-  __device__ void update_infection_dynamics(const uint32_t* __restrict__ tm_arr, uint64_t gi) {
+  __device__ __forceinline__ void update_infection_dynamics(const uint32_t* __restrict__ tm_arr, uint64_t gi) {
     const uint32_t s = state();
     if (s == ST_S) {
       if (on_susceptible()) {
@@ -227,7 +307,7 @@ struct Agent {
   }
 
   // citizen/mod.rs:257-349
-  __device__ void perform_movements(uint32_t hd, const uint32_t* __restrict__ tm_arr, uint64_t gi) {
+  __device__ __forceinline__ void perform_movements(uint32_t hd, const uint32_t* __restrict__ tm_arr, uint64_t gi) {
     const uint32_t s = ws();
     if (s == WS_NORMAL || s == WS_ESSENTIAL) {
       if (hd == 7 || hd == 17) {
@@ -279,7 +359,7 @@ struct Agent {
   }
 
   // citizen/mod.rs:351-365 with allocation_map.rs:144-150
-  __device__ void hospitalize() {
+  __device__ __forceinline__ void hospitalize() {
     if ((w & F_HOSP) || state() != ST_I || sev() != SEV_SEVERE) return;
     if (!((D.hosp_window >> window_of(D, w)) & 1u)) return;
     const uint32_t fv = S.first_vacant;
@@ -297,7 +377,7 @@ struct Agent {
   }
 
   // citizen/mod.rs:397-413 + default_disease_handler.rs:88-102
-  __device__ void deceased() {
+  __device__ __forceinline__ void deceased() {
     if (state() == ST_I) {
       const uint32_t day = fld(w, B_DAY, M_DAY), sv = sev();
       uint32_t ns = ST_I;
@@ -427,7 +507,7 @@ __global__ void k_step(DevView v) {
     if (a.tx != a.x || a.ty != a.y) {
       a.w |= F_MOVE;
       v.tgt[gi] = static_cast<uint32_t>(a.tx) | (static_cast<uint32_t>(a.ty) << 16);
-      const uint64_t ci = D.cell_off + static_cast<uint64_t>(a.ty) * D.stride + static_cast<uint32_t>(a.tx);
+      const uint64_t ci = D.cell_off + cell_index(D, a.tx, a.ty);
       atomicMin(&v.claim[ci], claim_key(h, local));
     }
     if (grid_byte(D, a.w) != grid_byte(D, w0)) a.w |= F_DIRTY;
@@ -494,7 +574,11 @@ __global__ void k_commit(DevView v) {
   const uint32_t local = first_lane(bm.first) + threadIdx.x;
   if (local < D.n) {
     const uint64_t gi = D.agent_off + local;
+    // st, pos and tgt are loaded together so a mover pays two dependent round trips
+    // (these three, then its claim), not four.
     const uint32_t w = v.st[gi];
+    const uint32_t p = v.pos[gi];
+    const uint32_t t = v.tgt[gi];
     uint32_t nw = w;
     if (sweep) {
       if ((sweep & SWEEP_VACC) && fld(w, B_STATE, M_STATE) == ST_S) {  // allocation_map.rs:381-387
@@ -506,13 +590,11 @@ __global__ void k_commit(DevView v) {
     }
     if (w & (F_MOVE | F_DIRTY)) {
       uint8_t* occ = v.occ + D.cell_off;
-      const uint32_t p = v.pos[gi];
-      const uint64_t old_ci = static_cast<uint64_t>(p >> 16) * D.stride + (p & 0xFFFFu);
+      const uint64_t old_ci = cell_index(D, static_cast<int32_t>(p & 0xFFFFu), static_cast<int32_t>(p >> 16));
       const uint8_t byte = grid_byte(D, w);
       bool moved = false;
       if (w & F_MOVE) {
-        const uint32_t t = v.tgt[gi];
-        const uint64_t ci = static_cast<uint64_t>(t >> 16) * D.stride + (t & 0xFFFFu);
+        const uint64_t ci = cell_index(D, static_cast<int32_t>(t & 0xFFFFu), static_cast<int32_t>(t >> 16));
         if (v.claim[D.cell_off + ci] == claim_key(h, local)) {
           occ[old_ci] = 0;
           occ[ci] = byte;
@@ -547,7 +629,7 @@ __global__ void k_hospital_scan(DevView v, uint32_t cells_per_thread) {
     const uint32_t c = first + k * blockDim.x + threadIdx.x;
     if (c >= ncell) break;
     const uint32_t cx = static_cast<uint32_t>(hr.x0) + c % wd, cy = static_cast<uint32_t>(hr.y0) + c / wd;
-    if (occ[static_cast<uint64_t>(cy) * D.stride + cx] == 0) { best = c; break; }
+    if (occ[cell_index(D, static_cast<int32_t>(cx), static_cast<int32_t>(cy))] == 0) { best = c; break; }
   }
   // wavefront min then one atomic per wave
   for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -579,7 +661,7 @@ hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t strea
 
 hipError_t launch_epilogue(const DevView& v, const LaunchShape& s, hipStream_t stream) {
   if (s.n_sims == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_epilogue, dim3(s.n_sims), dim3(256), 0, stream, v);
+  hipLaunchKernelGGL(k_epilogue, dim3(s.n_sims), dim3(1024), 0, stream, v);
   return hipGetLastError();
 }
 

@@ -8,9 +8,12 @@
 //   tm[i]   Exposed{at_hour} / Pre{at_hour}
 //   wsa[i]  HospitalStaff{work_start_at}      (read only for the 0.2% staff)
 //   tgt[i]  the cell chosen this hour          (step -> commit hand-off)
-// The grid is a dense (stride x (g+1)) byte array: 0 vacant, 1 occupied by an agent
-// that cannot infect, 2 / 3 occupied by an infectious agent in the regular / high
-// transmission window. Claims are a parallel u64 array keyed by (hour, agent id).
+// The grid is a dense byte array: 0 vacant, 1 occupied by an agent that cannot infect,
+// 2 / 3 occupied by an infectious agent in the regular / high transmission window. Cell
+// (x, y) lives at (y + kPadY) * pitch + x + kPadX: a zero border of kPadX columns / kPadY
+// rows (and pitch >= stride + 9, a multiple of 4) lets every kernel read the 5x5
+// neighbourhood of any cell as ten aligned dwords without bounds checks. Claims are a
+// parallel u64 array with the same indexing, keyed by (hour, agent id).
 #pragma once
 #include <cstdint>
 
@@ -61,11 +64,19 @@ struct Rect {
   int32_t x0, y0, x1, y1;
 };
 
+constexpr uint32_t kPadX = 4, kPadY = 2;
+inline uint32_t padded_pitch(uint32_t stride) { return (stride + 9u + 3u) & ~3u; }
+inline uint64_t padded_cells(uint32_t stride, uint32_t height) {
+  return static_cast<uint64_t>(padded_pitch(stride)) * (height + 2u * kPadY);
+}
+
 // Read-only per-simulation descriptor (device copy built once at create).
 struct SimDesc {
   uint32_t n;          // agents
   uint32_t g;          // grid_size
-  uint32_t stride;     // grid row pitch (>= g+1)
+  uint32_t stride;     // logical grid width (>= g+1)
+  uint32_t pitch;      // padded row pitch of the grid / claim arrays
+  uint32_t height;     // logical grid height (g+1)
   uint32_t hours;      // config hours: the loop runs 1..hours-1
   uint64_t agent_off;  // first agent in the concatenated SoA
   uint64_t cell_off;   // first cell in the concatenated grid / claim arrays
