@@ -1,17 +1,21 @@
 #!/usr/bin/env python3
 """Throughput of the per-hour agent step on one MI355X.
 
-Workload (default): 1,000,000 agents on a 2000x2000 grid, 70% working, 20% public
+Headline workload: 1,000,000 agents on a 2000x2000 grid, 70% working, 20% public
 transport, 10 initial exposed, 2,399 simulated hours, seed 42, deterministic mode
 (tests/golden/configs/cfg2_1m.json). A "step" is one simulated hour of the hot path over
-all agents (hour-of-day-0 hospital scan, agent step, commit, counts, interventions).
+all agents (hour-of-day-0 hospital scan, agent step, epilogue with counts and
+interventions, commit).
 
-    python bench.py [--gpus 1] [--steps 2399] [--warmup 240]
+    python bench.py [--gpus 1] [--steps 2399] [--warmup 240] [--skip-extra]
 
 Prints one JSON line. `value` is agent-hours/s = agents x hours stepped / wall seconds of
-the hour loop, inputs resident in HBM (population synthesis and upload happen before the
-timed region, as the reference's timer starts after Epidemiology::new,
-engine/src/epidemiology_simulation.rs:212-272).
+the hour loop with inputs resident in HBM (population synthesis and upload happen before
+the timed region, as the reference's timer starts after Epidemiology::new,
+engine/src/epidemiology_simulation.rs:212-272). Two further workloads are reported in the
+same line: 64 mixed replicates in one batched launch (replicates/s,
+tests/golden/configs/cfg4_replicates.json) and 10M agents with lockdown, hospital growth
+and daily vaccination over 4,799 hours (tests/golden/configs/cfg3_10m_interventions.json).
 """
 import argparse
 import json
@@ -22,7 +26,7 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-DEFAULT_CONFIG = os.path.join(ROOT, "tests", "golden", "configs", "cfg2_1m.json")
+CONFIGS = os.path.join(ROOT, "tests", "golden", "configs")
 HBM_PEAK_GBS = 8000.0
 
 
@@ -32,46 +36,51 @@ def algorithmic_bytes_per_hour(n, g):
     return 32.0 * n + 8.0 * g * g
 
 
-def cpu_baseline(cfg, target_seconds):
-    """The CPU oracle (single-threaded C restatement of the reference step) on the first
-    hours of the same workload, bounded to roughly target_seconds."""
+def cpu_threads():
+    # the GPU box's CPU share for one GPU is 16 cores
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def oracle_rate(cfgs, target_seconds, seeds=None, threads=None):
+    """CPU oracle (C restatement of the reference step: per-agent map over the
+    start-of-hour snapshot on `threads` threads, sequential merge) on the first hours of
+    the given configs, bounded to about target_seconds. Returns (agent-hours/s, text)."""
     from oracle import epicpu as O
-    n = cfg["population"]["Auto"]["number_of_agents"]
-    t0 = time.perf_counter()
-    sim = O.OracleSim(cfg)
-    init_s = time.perf_counter() - t0
-    hours = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < target_seconds and hours < cfg["hours"] - 1:
-        rows, stopped = sim.step(4)
-        hours += len(rows)
-        if stopped:
-            break
-    dt = time.perf_counter() - t0
-    sim.close()
-    return {"value": n * hours / dt, "unit": "agent-hours/s", "cores": 1, "kind": "port",
-            "sample": "oracle/epicpu.c, hours 1..%d of this workload (%.1f s; init %.1f s excluded)"
-                      % (hours, dt, init_s)}
+    threads = threads or cpu_threads()
+    agent_hours = 0
+    spent = 0.0
+    hours_by = []
+    for i, cfg in enumerate(cfgs):
+        sim = O.OracleSim(cfg, seed=None if seeds is None else seeds[i])
+        sim.set_threads(threads)
+        n = cfg["population"]["Auto"]["number_of_agents"]
+        budget = target_seconds / len(cfgs)
+        t0 = time.perf_counter()
+        h = 0
+        while time.perf_counter() - t0 < budget and h < cfg["hours"] - 1:
+            rows, stopped = sim.step(4)
+            h += len(rows)
+            if stopped:
+                break
+        dt = time.perf_counter() - t0
+        sim.close()
+        agent_hours += n * h
+        spent += dt
+        hours_by.append(h)
+    return agent_hours / spent, "oracle/epicpu.c, %d threads, first %s hours of %d config(s), %.1f s" % (
+        threads, "/".join(str(h) for h in hours_by[:4]) + ("/..." if len(hours_by) > 4 else ""), len(cfgs), spent)
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2399, help="simulated hours timed")
-    ap.add_argument("--warmup", type=int, default=240, help="untimed hours on a separate instance")
-    ap.add_argument("--config", default=DEFAULT_CONFIG)
-    ap.add_argument("--block", type=int, default=0, help="threads per agent block (0 = default)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (one region per GPU; "
-              "see DESIGN.md for how the step would shard).", file=sys.stderr)
-        sys.exit(2)
-
+def timed_steps(handle, hours):
     import torch
-    import epirust_amd as G
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows, stopped = handle.step(hours)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, rows
 
+
+def bench_headline(args, G):
     with open(args.config) as f:
         cfg = json.load(f)
     n = cfg["population"]["Auto"]["number_of_agents"]
@@ -89,27 +98,91 @@ def main():
         w.step(min(args.warmup, run_hours))
         w.close()
     sims = [make() for _ in range(max(1, math.ceil(args.steps / run_hours)))]
-
+    import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hours_done = 0
-    remaining = args.steps
+    hours_done, remaining = 0, args.steps
     for s in sims:
-        rows, stopped = s.step(min(remaining, run_hours))
+        k = min(remaining, run_hours)
+        rows, _ = s.step(k)
         hours_done += len(rows)
-        remaining -= min(remaining, run_hours)
+        remaining -= k
         if remaining <= 0:
             break
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     for s in sims:
         s.close()
-    value = n * hours_done / elapsed
-
-    # kernel durations from HIP events on the simulation's stream, over a separate run
     p = make()
     prof = p.profile(min(args.steps, run_hours))
     p.close()
+    return cfg, n, g, run_hours, hours_done, elapsed, prof
+
+
+def bench_replicates(G, cpu_seconds):
+    from epirust_amd import autotune
+    with open(os.path.join(CONFIGS, "cfg4_replicates.json")) as f:
+        cfgs = json.load(f)
+    block = autotune.tuned_block(cfgs)
+    b = G.Batch(cfgs)
+    if block:
+        b.set_block_size(block)
+    dt, rows = timed_steps(b, 2399)
+    b.close()
+    agent_hours = sum(c["population"]["Auto"]["number_of_agents"] * len(r) for c, r in zip(cfgs, rows))
+    out = {"metric": "replicates/s @64 mixed", "value": len(cfgs) / dt, "unit": "replicates/s",
+           "seconds": dt, "agent_hours_per_s": agent_hours / dt, "replicates": len(cfgs),
+           "total_agents": sum(c["population"]["Auto"]["number_of_agents"] for c in cfgs),
+           "block": block or 256, "hours_per_replicate": 2399}
+    if cpu_seconds > 0:
+        rate, sample = oracle_rate(cfgs[:4], cpu_seconds)
+        full = sum(c["population"]["Auto"]["number_of_agents"] * 2399 for c in cfgs)
+        out["cpu_baseline"] = {"value": len(cfgs) / (full / rate), "unit": "replicates/s", "cores": cpu_threads(),
+                               "kind": "port", "sample": sample + "; replicates run serially, projected "
+                               "from agent-hours/s over the batch's %.3g agent-hours" % full}
+    return out
+
+
+def bench_interventions(G, cpu_seconds):
+    with open(os.path.join(CONFIGS, "cfg3_10m_interventions.json")) as f:
+        cfg = json.load(f)
+    s = G.Simulation(cfg)
+    dt, rows = timed_steps(s, cfg["hours"] - 1)
+    ev = s.events()
+    s.close()
+    n = cfg["population"]["Auto"]["number_of_agents"]
+    out = {"metric": "agent-hours/s @10M agents with interventions", "value": n * len(rows) / dt,
+           "unit": "agent-hours/s", "hours": len(rows), "seconds": dt,
+           "events": {"vaccination": sum(1 for e in ev if e[1] == 0), "lockdown": sum(1 for e in ev if e[1] in (1, 2)),
+                      "build_new_hospital": sum(1 for e in ev if e[1] == 3)},
+           "peak_infected": int(rows[:, 3].max()) if len(rows) else 0}
+    if cpu_seconds > 0:
+        rate, sample = oracle_rate([cfg], cpu_seconds)
+        out["cpu_baseline"] = {"value": rate, "unit": "agent-hours/s", "cores": cpu_threads(), "kind": "port",
+                               "sample": sample}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2399, help="simulated hours timed")
+    ap.add_argument("--warmup", type=int, default=240, help="untimed hours on a separate instance")
+    ap.add_argument("--config", default=os.path.join(CONFIGS, "cfg2_1m.json"))
+    ap.add_argument("--block", type=int, default=0, help="threads per agent block (0 = default)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-extra", action="store_true", help="only the headline workload")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (one region per GPU; "
+              "DESIGN.md describes how the step would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import epirust_amd as G
+
+    cfg, n, g, run_hours, hours_done, elapsed, prof = bench_headline(args, G)
+    value = n * hours_done / elapsed
     scan_ms, n_scan = prof["hospital_scan"]
     step_ms, n_step = prof["step"]
     commit_ms, _ = prof["commit"]
@@ -120,7 +193,6 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_hour")
-
     result = {
         "metric": "agent-hours/s @1M agents",
         "value": value,
@@ -137,13 +209,20 @@ def main():
         "config": {"workload": "EpiRust standalone hour step, %d agents, %dx%d grid, %d hours, seed %s, "
                                "deterministic mode" % (n, g, g, run_hours, cfg.get("seed", 0)),
                    "agents": n, "grid_size": g, "hours": run_hours,
-                   "kernel_ms": {"hospital_scan": scan_ms, "step": step_ms, "commit": commit_ms}},
+                   "kernel_ms": {"hospital_scan": scan_ms, "step_and_epilogue": step_ms, "commit": commit_ms}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "hour step = k_step + k_commit (+ k_hospital_scan once a day)"},
+                     "kernel": "one simulated hour: k_step + k_epilogue + k_commit (+ k_hospital_scan once a day)",
+                     "algorithmic_bytes_per_hour": algorithmic_bytes_per_hour(n, g)},
     }
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    cpu_s = 0.0 if args.no_cpu_baseline else args.cpu_seconds
+    if cpu_s > 0:
+        rate, sample = oracle_rate([cfg], cpu_s)
+        result["cpu_baseline"] = {"value": rate, "unit": "agent-hours/s", "cores": cpu_threads(), "kind": "port",
+                                  "sample": sample}
+    if not args.skip_extra:
+        result["replicates"] = bench_replicates(G, cpu_s / 3)
+        result["interventions_10m"] = bench_interventions(G, cpu_s / 3)
     print(json.dumps(result))
 
 

@@ -81,6 +81,20 @@ SYMBOLS = {
     "epi_last_error": (ctypes.c_char_p, []),
     "epi_destroy": (None, [ctypes.c_void_p]),
     "epi_cli_main": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_char_p)]),
+    "epi_mlp_create": (ctypes.c_int, [_P(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                      _P(ctypes.c_void_p)]),
+    "epi_mlp_param_count": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "epi_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
+    "epi_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
+    "epi_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "epi_mlp_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                          ctypes.c_float, _P(ctypes.c_float)]),
+    "epi_mlp_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                         ctypes.c_void_p, _P(ctypes.c_float)]),
+    "epi_mlp_last_error": (ctypes.c_char_p, []),
+    "epi_mlp_destroy": (None, [ctypes.c_void_p]),
+    "epi_gemm_bf16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int]),
 }
 
 

@@ -27,8 +27,8 @@ HOST = ["-I" + os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__"]
 # Packed FP32 instructions are kept out of every code object built here.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-HOST_SOURCES = ["config.cpp", "population.cpp", "engine.cpp", "cli.cpp"]
-DEVICE_SOURCES = ["kernels.hip"]
+HOST_SOURCES = ["config.cpp", "population.cpp", "engine.cpp", "cli.cpp", "mlp_engine.cpp"]
+DEVICE_SOURCES = ["kernels.hip", "mlp.hip"]
 
 
 def _run(cmd):

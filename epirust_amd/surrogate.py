@@ -1,0 +1,369 @@
+"""Learned surrogate for parameter sweeps of the per-hour agent step.
+
+An MLP (9 -> 4 x 512 GELU -> 100 days x 6 compartments) maps transmission, mobility and
+intervention parameters to the daily S/E/I/H/R/D fractions of a 100,000-agent region
+(632 x 632 grid, 2,400 hours). It is trained on ensembles that the batched GPU simulator
+generates, with the library's own bf16-MFMA forward/backward kernels (epi_mlp_*), and the
+plain simulation stays the fallback and the judge of its error:
+
+* inputs are Sobol points over the ranges in tests/golden/configs/cfg5_surrogate.json;
+* a daily sample is the row with hour % 24 == 1 (the reference's plotting rule,
+  engine/plot/plot.py:33); a run that stops early carries its last row forward;
+* peak infected = max over days of the infected fraction; final size = infected +
+  hospitalized + recovered + deceased on the last day (the reference's `totalinfected`,
+  engine/plot/update_total_infections.py:19);
+* two independently initialised networks are trained; their disagreement on peak and
+  final size is the predicted error, and a point whose predicted error exceeds the
+  threshold is simulated instead (32 seeds, batched).
+"""
+import copy
+import ctypes
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+import epirust_amd as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SPEC_PATH = os.path.join(ROOT, "tests", "golden", "configs", "cfg5_surrogate.json")
+BASE_PATH = os.path.join(ROOT, "tests", "golden", "configs", "cfg1_10k.json")
+WIDTHS = [9, 512, 512, 512, 512, 600]
+
+
+def load_spec(path=SPEC_PATH):
+    with open(path) as f:
+        return json.load(f)
+
+
+def sobol_points(n, seed=0, dims=9):
+    """Scrambled Sobol points in [0, 1)^dims."""
+    from scipy.stats import qmc
+    s = qmc.Sobol(d=dims, scramble=True, seed=seed)
+    m = int(math.ceil(math.log2(max(n, 2))))
+    return s.random_base2(m)[:n]
+
+
+def to_params(unit, spec):
+    lo = np.array([i["low"] for i in spec["inputs"]], dtype=np.float64)
+    hi = np.array([i["high"] for i in spec["inputs"]], dtype=np.float64)
+    return lo + np.asarray(unit, dtype=np.float64) * (hi - lo)
+
+
+def point_config(p, spec, base=None, seed=0):
+    """Reference-schema config for one parameter point (unscaled values)."""
+    if base is None:
+        with open(BASE_PATH) as f:
+            base = json.load(f)
+    c = copy.deepcopy(base)
+    n = spec["agents"]
+    c["population"]["Auto"] = {"number_of_agents": n, "working_percentage": float(p[3]),
+                               "public_transport_percentage": float(p[4])}
+    c["geography_parameters"]["grid_size"] = spec["grid_size"]
+    c["hours"] = spec["hours"]
+    c["disease"]["regular_transmission_rate"] = float(p[0])
+    c["disease"]["high_transmission_rate"] = float(p[1])
+    c["disease"]["death_rate"] = float(p[2])
+    c["starting_infections"] = {"exposed": int(round(p[8])), "infected_mild_asymptomatic": 0,
+                                "infected_mild_symptomatic": 0, "infected_severe": 0}
+    iv = [{"Lockdown": {"at_number_of_infections": int(round(p[5] * n)), "essential_workers_population": float(p[6])}}]
+    if p[7] > 0:
+        start = spec["vaccination_start_hour"]
+        iv += [{"Vaccinate": {"at_hour": start + 24 * k, "percent": float(p[7])}}
+               for k in range((spec["hours"] - start) // 24)]
+    c["interventions"] = iv
+    c["seed"] = int(seed)
+    return c
+
+
+def daily_curves(rows, n, days):
+    """(days, 6) compartment fractions from hourly rows (hour % 24 == 1), carried forward."""
+    out = np.zeros((days, 6), dtype=np.float32)
+    last = None
+    by_hour = {int(r[0]): r for r in rows}
+    for d in range(days):
+        h = 24 * d + 1
+        r = by_hour.get(h)
+        if r is None:
+            r = rows[-1] if len(rows) and rows[-1][0] < h else last
+        if r is None:
+            continue
+        last = r
+        out[d] = np.asarray(r[1:7], dtype=np.float64) / n
+    return out
+
+
+def peak_and_final(curves):
+    """Peak infected fraction and final size from (..., days, 6) curves."""
+    c = np.asarray(curves)
+    peak = c[..., :, 2].max(axis=-1)
+    final = c[..., -1, 2] + c[..., -1, 3] + c[..., -1, 4] + c[..., -1, 5]
+    return peak, final
+
+
+def simulate(params, seeds, spec, batch=128, block=None, progress=None):
+    """Daily curves (len(params), days, 6) from the batched GPU simulator."""
+    days = spec["days"]
+    n = spec["agents"]
+    with open(BASE_PATH) as f:
+        base = json.load(f)
+    out = np.zeros((len(params), days, 6), dtype=np.float32)
+    for s in range(0, len(params), batch):
+        cfgs = [point_config(params[i], spec, base, seeds[i]) for i in range(s, min(s + batch, len(params)))]
+        b = G.Batch(cfgs)
+        if block:
+            b.set_block_size(block)
+        rows, _ = b.step(spec["hours"] - 1)
+        b.close()
+        for j, r in enumerate(rows):
+            out[s + j] = daily_curves(r, n, days)
+        if progress:
+            progress(min(s + batch, len(params)), len(params))
+    return out
+
+
+class MLP:
+    """The epi_mlp handle (bf16-MFMA MLP with Adam) over ctypes."""
+
+    def __init__(self, widths=WIDTHS, seed=0, max_batch=1024):
+        self.widths = list(widths)
+        self.h = ctypes.c_void_p()
+        w = (ctypes.c_uint32 * len(widths))(*widths)
+        rc = G.lib().epi_mlp_create(w, len(widths), seed, max_batch, ctypes.byref(self.h))
+        if rc:
+            raise G.EpiError(rc, G.lib().epi_mlp_last_error().decode())
+        self.max_batch = max_batch
+
+    def close(self):
+        if self.h:
+            G.lib().epi_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        n = G.lib().epi_mlp_param_count(self.h)
+        p = np.zeros(n, dtype=np.float32)
+        G.lib().epi_mlp_get_params(self.h, p.ctypes.data, n)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        rc = G.lib().epi_mlp_set_params(self.h, p.ctypes.data, len(p))
+        if rc:
+            raise G.EpiError(rc, G.lib().epi_mlp_last_error().decode())
+
+    def train_step(self, x, t, lr):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = ctypes.c_float()
+        rc = G.lib().epi_mlp_train_step(self.h, x.ctypes.data, t.ctypes.data, len(x), lr, ctypes.byref(loss))
+        if rc:
+            raise G.EpiError(rc, G.lib().epi_mlp_last_error().decode())
+        return loss.value
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((len(x), self.widths[-1]), dtype=np.float32)
+        for s in range(0, len(x), self.max_batch):
+            k = min(self.max_batch, len(x) - s)
+            rc = G.lib().epi_mlp_forward(self.h, x[s:].ctypes.data, k, y[s:].ctypes.data)
+            if rc:
+                raise G.EpiError(rc, G.lib().epi_mlp_last_error().decode())
+        return y
+
+
+def train_mlp(X, Y, seed, epochs=60, batch=512, lr=1e-3, widths=WIDTHS, log=None):
+    """Fits one network on standardized targets; returns (mlp, loss history)."""
+    rng = np.random.default_rng(seed)
+    m = MLP(widths, seed=seed, max_batch=max(batch, 1024))
+    hist = []
+    steps_per_epoch = max(1, len(X) // batch)
+    total = epochs * steps_per_epoch
+    step = 0
+    for ep in range(epochs):
+        perm = rng.permutation(len(X))
+        acc = 0.0
+        for k in range(steps_per_epoch):
+            idx = perm[k * batch:(k + 1) * batch]
+            # cosine decay of the learning rate
+            cur = lr * 0.5 * (1 + math.cos(math.pi * step / total))
+            acc += m.train_step(X[idx], Y[idx], max(cur, lr * 0.02))
+            step += 1
+        hist.append(acc / steps_per_epoch)
+        if log:
+            log(ep, hist[-1])
+    return m, hist
+
+
+class Surrogate:
+    """Two-network ensemble with standardized I/O and a simulation fallback."""
+
+    def __init__(self, spec, members, y_mean, y_std):
+        self.spec = spec
+        self.members = members
+        self.y_mean = y_mean
+        self.y_std = y_std
+
+    @classmethod
+    def fit(cls, unit_x, curves, spec, epochs=60, seeds=(1, 2), log=None):
+        days = spec["days"]
+        Y = curves.reshape(len(curves), days * 6).astype(np.float64)
+        y_mean = Y.mean(axis=0)
+        y_std = Y.std(axis=0) + 1e-4
+        Yn = ((Y - y_mean) / y_std).astype(np.float32)
+        X = np.asarray(unit_x, dtype=np.float32)
+        members = []
+        for s in seeds:
+            m, hist = train_mlp(X, Yn, seed=s, epochs=epochs, log=log)
+            members.append((m, hist))
+        return cls(spec, [m for m, _ in members], y_mean, y_std), [h for _, h in members]
+
+    def predict_members(self, unit_x):
+        days = self.spec["days"]
+        out = []
+        for m in self.members:
+            yn = m.forward(np.asarray(unit_x, dtype=np.float32)).astype(np.float64)
+            y = yn * self.y_std + self.y_mean
+            out.append(np.clip(y, 0.0, 1.0).reshape(len(unit_x), days, 6))
+        return out
+
+    def predict(self, unit_x):
+        """Mean curves and the predicted relative error (member disagreement) of peak and final size."""
+        ys = self.predict_members(unit_x)
+        mean = np.mean(ys, axis=0)
+        pk, fs = zip(*[peak_and_final(y) for y in ys])
+        pk, fs = np.array(pk), np.array(fs)
+        mp, mf = peak_and_final(mean)
+        err = np.maximum(np.abs(pk[0] - pk[1]) / np.maximum(mp, 1e-9), np.abs(fs[0] - fs[1]) / np.maximum(mf, 1e-9))
+        return mean, err
+
+    def sweep(self, unit_x, seeds_per_point, threshold=None, block=None):
+        """Curves for every point: the surrogate where its predicted error is within the
+        threshold, else the mean of a seeds_per_point simulation ensemble. Returns
+        (curves, fallback mask, seconds)."""
+        threshold = self.spec["max_relative_error"] if threshold is None else threshold
+        t0 = time.perf_counter()
+        curves, err = self.predict(unit_x)
+        fb = err > threshold
+        idx = np.nonzero(fb)[0]
+        if len(idx):
+            params = to_params(unit_x[idx], self.spec)
+            rep = np.repeat(params, seeds_per_point, axis=0)
+            seeds = np.tile(np.arange(1, seeds_per_point + 1), len(idx))
+            sims = simulate(rep, seeds, self.spec, block=block)
+            curves[idx] = sims.reshape(len(idx), seeds_per_point, self.spec["days"], 6).mean(axis=1)
+        return curves, fb, time.perf_counter() - t0
+
+
+def nearest_neighbour(train_x, train_curves, query_x):
+    """The no-network method: each query takes the curve of its nearest training point."""
+    tx = np.asarray(train_x, dtype=np.float64)
+    out = np.zeros((len(query_x),) + train_curves.shape[1:], dtype=np.float32)
+    for i, q in enumerate(np.asarray(query_x, dtype=np.float64)):
+        out[i] = train_curves[np.argmin(((tx - q) ** 2).sum(axis=1))]
+    return out
+
+
+def relative_errors(pred_curves, ref_curves):
+    pp, pf = peak_and_final(pred_curves)
+    rp, rf = peak_and_final(ref_curves)
+    return np.abs(pp - rp) / np.maximum(rp, 1e-9), np.abs(pf - rf) / np.maximum(rf, 1e-9)
+
+
+def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=60, block=None, log=print):
+    """Generate data, train, evaluate; writes surrogate_weights.npz and surrogate_metrics.json."""
+    spec = load_spec()
+    n_train = n_train or spec["train_points"]
+    n_eval = n_eval or spec["eval_points"]
+    eval_seeds = eval_seeds or spec["eval_seeds"]
+    os.makedirs(out_dir, exist_ok=True)
+    metrics = {"train_points": n_train, "eval_points": n_eval, "eval_seeds": eval_seeds, "epochs": epochs,
+               "agents": spec["agents"], "grid_size": spec["grid_size"], "hours": spec["hours"]}
+
+    def prog(label):
+        last = [0.0]
+
+        def f(done, total):
+            if time.perf_counter() - last[0] > 20 or done == total:
+                last[0] = time.perf_counter()
+                log("[surrogate] %s %d/%d" % (label, done, total))
+        return f
+
+    ux = sobol_points(n_train, seed=0)
+    t0 = time.perf_counter()
+    curves = simulate(to_params(ux, spec), 1_000_000 + np.arange(n_train), spec, block=block, progress=prog("train sims"))
+    metrics["train_sim_seconds"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sur, hists = Surrogate.fit(ux, curves, spec, epochs=epochs,
+                               log=lambda ep, l: log("[surrogate] epoch %d loss %.5f" % (ep, l)) if ep % 10 == 0 else None)
+    metrics["train_seconds"] = time.perf_counter() - t0
+    metrics["final_train_loss"] = [h[-1] for h in hists]
+
+    ex = sobol_points(n_eval, seed=12345)
+    params = to_params(ex, spec)
+    rep = np.repeat(params, eval_seeds, axis=0)
+    seeds = np.tile(np.arange(1, eval_seeds + 1), n_eval)
+    t0 = time.perf_counter()
+    ens = simulate(rep, seeds, spec, block=block, progress=prog("eval sims")).reshape(n_eval, eval_seeds, spec["days"], 6)
+    plain_seconds = time.perf_counter() - t0
+    ref = ens.mean(axis=1)
+    metrics["plain_sweep_seconds"] = plain_seconds
+    metrics["plain_points_per_s"] = n_eval / plain_seconds
+
+    t0 = time.perf_counter()
+    pred, err_hat = sur.predict(ex)
+    metrics["surrogate_only_seconds"] = time.perf_counter() - t0
+    ep_, ef_ = relative_errors(pred, ref)
+    nn = nearest_neighbour(ux, curves, ex)
+    np_, nf_ = relative_errors(nn, ref)
+    ok = (ep_ <= spec["max_relative_error"]) & (ef_ <= spec["max_relative_error"])
+    ok_nn = (np_ <= spec["max_relative_error"]) & (nf_ <= spec["max_relative_error"])
+    metrics.update({
+        "surrogate_median_rel_err_peak": float(np.median(ep_)), "surrogate_median_rel_err_final": float(np.median(ef_)),
+        "surrogate_fraction_within": float(ok.mean()),
+        "nn_median_rel_err_peak": float(np.median(np_)), "nn_median_rel_err_final": float(np.median(nf_)),
+        "nn_fraction_within": float(ok_nn.mean()),
+    })
+    # seed-to-seed spread of the reference ensembles (how noisy a single simulation is)
+    sp, sf = peak_and_final(ens)
+    metrics["ensemble_cv_peak_median"] = float(np.median(sp.std(axis=1) / np.maximum(sp.mean(axis=1), 1e-9)))
+    metrics["ensemble_cv_final_median"] = float(np.median(sf.std(axis=1) / np.maximum(sf.mean(axis=1), 1e-9)))
+
+    curves_fb, fb, sweep_seconds = sur.sweep(ex, eval_seeds, block=block)
+    efp, eff = relative_errors(curves_fb, ref)
+    okf = (efp <= spec["max_relative_error"]) & (eff <= spec["max_relative_error"])
+    metrics.update({
+        "fallback_fraction": float(fb.mean()),
+        "surrogate_sweep_seconds": sweep_seconds,
+        "surrogate_points_per_s": n_eval / sweep_seconds,
+        "speedup_vs_plain_sweep": (n_eval / sweep_seconds) / (n_eval / plain_seconds),
+        "with_fallback_fraction_within": float(okf.mean()),
+        "with_fallback_max_rel_err_surrogate_points": float(np.max(np.maximum(efp, eff)[~fb])) if (~fb).any() else None,
+    })
+    flat = [m.params() for m in sur.members]
+    np.savez_compressed(os.path.join(out_dir, "surrogate_weights.npz"), member0=flat[0], member1=flat[1],
+                        y_mean=sur.y_mean, y_std=sur.y_std)
+    with open(os.path.join(out_dir, "surrogate_metrics.json"), "w") as f:
+        json.dump(metrics, f, indent=1)
+    log(json.dumps(metrics))
+    return metrics
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default="gpurun_out/surrogate")
+    ap.add_argument("--train", type=int, default=0)
+    ap.add_argument("--eval", type=int, default=0)
+    ap.add_argument("--seeds", type=int, default=0)
+    ap.add_argument("--epochs", type=int, default=60)
+    a = ap.parse_args()
+    run_pipeline(a.out, a.train or None, a.eval or None, a.seeds or None, a.epochs,
+                 log=lambda s: print(s, flush=True))

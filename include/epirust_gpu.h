@@ -120,6 +120,28 @@ const char* epi_last_error(void);
 
 void epi_destroy(epi_sim* sim);
 
+/* ---- Learned surrogate (no reference counterpart: the reference only collates ensemble
+ * mean/std, engine/plot/models/EpiCurves.py:30-40). An MLP widths[0] -> ... -> widths[n-1]
+ * with GELU after every hidden layer and an identity output, bf16 weights/activations on
+ * MFMA with FP32 accumulation, MSE loss, Adam. Parameters are laid out W0 (K x N row-major),
+ * b0, W1, b1, ... as fp32. */
+typedef struct epi_mlp epi_mlp;
+int epi_mlp_create(const uint32_t* widths, uint32_t n_widths, uint64_t seed, uint32_t max_batch, epi_mlp** out);
+uint64_t epi_mlp_param_count(epi_mlp* mlp);
+int epi_mlp_get_params(epi_mlp* mlp, float* params, uint64_t n);
+int epi_mlp_set_params(epi_mlp* mlp, const float* params, uint64_t n);
+/* y (batch x widths[n-1]) = network(x (batch x widths[0])). */
+int epi_mlp_forward(epi_mlp* mlp, const float* x, uint32_t batch, float* y);
+/* One Adam step on the MSE of network(x) against t; loss (optional) gets the batch MSE. */
+int epi_mlp_train_step(epi_mlp* mlp, const float* x, const float* t, uint32_t batch, float lr, float* loss);
+/* Parameter gradients of the batch MSE (same layout as the parameters), for checks. */
+int epi_mlp_gradients(epi_mlp* mlp, const float* x, const float* t, uint32_t batch, float* grads, float* loss);
+const char* epi_mlp_last_error(void);
+void epi_mlp_destroy(epi_mlp* mlp);
+/* C (M x N, fp32) = A * B with bf16 inputs on the MFMA kernel; A is M x K row-major, or
+ * K x M when trans_a; B is K x N, or N x K when trans_b. Host buffers. */
+int epi_gemm_bf16(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K, int trans_a, int trans_b);
+
 /* Command-line entry equivalent to engine-app (engine-app/src/main.rs:60-87, 167-173):
  * -c/--config, -m/--mode (standalone only), -i/--id, -t/--threads (accepted, unused),
  * -o/--output-dir (default /tmp). Returns a process exit code. */

@@ -19,6 +19,7 @@
 #include "epicpu.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -246,7 +247,12 @@ struct epicpu_sim {
     int hosp_applied;
     event* events;
     uint32_t n_events, cap_events;
+    uint32_t threads;
 };
+
+void epicpu_set_threads(epicpu_sim* s, uint32_t threads) {
+    s->threads = threads < 1 ? 1 : (threads > 64 ? 64 : threads);
+}
 
 static inline uint32_t cell_index(const epicpu_sim* s, int32_t x, int32_t y) {
     return (uint32_t)y * s->stride + (uint32_t)x;
@@ -572,13 +578,36 @@ static void update_counts(uint32_t counts[7], const citizen* c) { /* counts.rs:1
     }
 }
 
+/* The parallel map of allocation_map.rs:82-92: every agent's routine reads only the
+   start-of-hour snapshot, so agents can be split over threads like rayon's par_iter. */
+typedef struct {
+    epicpu_sim* s;
+    uint32_t h, begin, end;
+} map_job;
+
+static void* map_range(void* arg) {
+    map_job* j = (map_job*)arg;
+    for (uint32_t i = j->begin; i < j->end; i++) {
+        j->s->upd[i] = j->s->ag[i];
+        j->s->tgt[i] = routine(j->s, &j->s->upd[i], j->s->pos[i], j->h);
+    }
+    return NULL;
+}
+
 /* allocation_map.rs:67-134 */
 static int simulate(epicpu_sim* s, uint32_t h) {
     for (int k = 1; k < 7; k++) s->counts[k] = 0;
-    for (uint32_t i = 0; i < s->n; i++) {
-        s->upd[i] = s->ag[i];
-        s->tgt[i] = routine(s, &s->upd[i], s->pos[i], h);
+    uint32_t nt = s->threads > 1 && s->n >= 4096 ? s->threads : 1;
+    map_job jobs[64];
+    pthread_t th[64];
+    uint32_t chunk = (s->n + nt - 1) / nt;
+    for (uint32_t k = 0; k < nt; k++) {
+        uint32_t b = k * chunk, e = b + chunk < s->n ? b + chunk : s->n;
+        jobs[k] = (map_job){s, h, b < s->n ? b : s->n, e};
     }
+    for (uint32_t k = 1; k < nt; k++) pthread_create(&th[k], NULL, map_range, &jobs[k]);
+    map_range(&jobs[0]);
+    for (uint32_t k = 1; k < nt; k++) pthread_join(th[k], NULL);
     /* Sequential merge. Iterating in id order makes "first inserted wins" equal to
        "lowest agent id wins". Targets are vacant in the snapshot or the agent's own
        cell, so a loser's old cell is always free in upcoming. */

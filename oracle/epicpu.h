@@ -89,6 +89,8 @@ typedef struct epicpu_sim epicpu_sim;
 /* Returns NULL on error and writes the message (mirroring the reference panic text). */
 epicpu_sim* epicpu_create(const epicpu_config* cfg, char* err, size_t errlen);
 void epicpu_destroy(epicpu_sim* s);
+/* Threads for the per-agent map (the merge stays sequential); results do not depend on it. */
+void epicpu_set_threads(epicpu_sim* s, uint32_t threads);
 /* Runs up to n_hours hours; rows gets 7 u32 per hour (hour,S,E,I,H,R,D). */
 int epicpu_step(epicpu_sim* s, uint32_t n_hours, uint32_t* rows, uint32_t* n_written, int* stopped);
 void epicpu_read_agents(const epicpu_sim* s, epicpu_agents* out);

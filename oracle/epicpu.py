@@ -89,6 +89,7 @@ def lib():
         L.epicpu_create.restype = ctypes.c_void_p
         L.epicpu_create.argtypes = [_P(EpicpuConfig), ctypes.c_char_p, ctypes.c_size_t]
         L.epicpu_destroy.argtypes = [ctypes.c_void_p]
+        L.epicpu_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
         L.epicpu_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_uint32), _P(ctypes.c_uint32),
                                   _P(ctypes.c_int)]
         L.epicpu_read_agents.argtypes = [ctypes.c_void_p, _P(EpicpuAgents)]
@@ -188,6 +189,9 @@ class OracleSim:
         if not self.h:
             raise RuntimeError(err.value.decode())
         self.n = self.cfg.number_of_agents
+
+    def set_threads(self, threads):
+        lib().epicpu_set_threads(self.h, threads)
 
     def close(self):
         if self.h:

@@ -168,3 +168,35 @@ def test_cfg2_full_run_rows_match_oracle_fixture():
     a = gpu.agents()
     from tests.golden.make_golden import state_digest
     assert state_digest(a) == str(gold["digest"])
+
+
+def test_cfg3_10m_first_day_bit_exact():
+    """10,000,000 agents on 6000x6000 with interventions configured: 30 hours agent by agent."""
+    cfg = json.load(open(os.path.join(HERE, "golden", "configs", "cfg3_10m_interventions.json")))
+    run_lockstep(cfg, 30, "cfg3", every=15)
+
+
+def test_cfg4_batched_replicates_match_their_own_runs():
+    """The 64-replicate batch in one launch: 8 of its replicates compared with their own
+    oracle runs (rows and agents) after 72 hours; all 64 checked for count invariants
+    over the full 2,399 hours."""
+    cfgs = json.load(open(os.path.join(HERE, "golden", "configs", "cfg4_replicates.json")))
+    say("[parity] cfg4 batch of %d, first 72 hours" % len(cfgs))
+    batch = G.Batch(cfgs)
+    rows, stopped = batch.step(72)
+    for i in range(0, 64, 8):
+        cpu = O.OracleSim(cfgs[i])
+        c_rows, c_stop = cpu.step(72)
+        assert np.array_equal(rows[i], c_rows), "replicate %d rows" % i
+        assert_agents_equal(batch.agents(i), cpu.agents(), "replicate %d" % i)
+        cpu.close()
+    say("[parity] cfg4 full run invariants")
+    rest, stopped = batch.step(2399 - 72)
+    for i, c in enumerate(cfgs):
+        r = np.concatenate([rows[i], rest[i]]) if len(rest[i]) else rows[i]
+        n = c["population"]["Auto"]["number_of_agents"]
+        assert (r[:, 1:].sum(axis=1) == n).all()
+        assert (np.diff(r[:, 1].astype(np.int64)) <= 0).all()          # S never grows
+        assert (np.diff(r[:, 5].astype(np.int64) + r[:, 6]) >= 0).all()  # R + D never shrink
+        assert (np.diff(r[:, 0].astype(np.int64)) == 1).all() and r[0, 0] == 1
+        assert stopped[i]
