@@ -163,6 +163,27 @@ def bench_interventions(G, cpu_seconds):
     return out
 
 
+def profile_driver(args, G):
+    import ctypes
+    with open(args.config) as f:
+        cfg = json.load(f)
+    s = G.Simulation(cfg)
+    if args.block:
+        s.set_block_size(args.block)
+    prof = s.profile(args.profile_hours)
+    s.close()
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 64 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"profile_hours": prof["hours"], "kernel_ms": prof, "calibration_copy_bytes": nbytes}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +194,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--profile-hours", type=int, default=0,
+                    help="profiling driver for rocprofv3: run this many hours as separate kernel dispatches "
+                         "(no graphs, no torch), then one 64 MiB device-to-device copy as the known-size "
+                         "calibration dispatch for FETCH_SIZE / WRITE_SIZE, and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (one region per GPU; "
@@ -180,6 +205,10 @@ def main():
         sys.exit(2)
 
     import epirust_amd as G
+
+    if args.profile_hours:
+        profile_driver(args, G)
+        return
 
     cfg, n, g, run_hours, hours_done, elapsed, prof = bench_headline(args, G)
     value = n * hours_done / elapsed

@@ -6,9 +6,11 @@
 // `population` and each `interventions` entry are externally tagged enums.
 #include "config.hpp"
 
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdlib>
+#include <fstream>
 #include <map>
 #include <memory>
 
@@ -271,8 +273,15 @@ bool parse_config(const std::string& text, Config& cfg, std::string& err) {
         get_u32(c, *body, "number_of_agents", cfg.number_of_agents);
         get_f64(c, *body, "public_transport_percentage", cfg.public_transport_percentage);
         get_f64(c, *body, "working_percentage", cfg.working_percentage);
-      } else if (tag == "Csv") {
-        c.fail("Population::Csv is not supported by this engine; use Population::Auto");
+      } else if (tag == "Csv" && body && body->kind == JValue::Obj) {
+        const JValue* file = need(c, *body, "file");
+        const JValue* cols = need(c, *body, "cols");
+        if (file && file->kind != JValue::Str) c.fail("invalid type for `file`, expected a string");
+        else if (cols && cols->kind != JValue::Arr) c.fail("invalid type for `cols`, expected a sequence");
+        else if (file) {
+          cfg.csv_population = true;
+          cfg.csv_file = file->str;
+        }
       } else {
         c.fail("unknown variant `" + tag + "`, expected `Csv` or `Auto`");
       }
@@ -359,6 +368,93 @@ bool parse_config(const std::string& text, Config& cfg, std::string& err) {
     else { cfg.seed = sd->uval; cfg.has_seed = true; }
   }
   if (!c.err.empty()) { err = c.err; return false; }
+  return true;
+}
+
+namespace {
+
+// One CSV record split on commas; a field wrapped in double quotes may contain commas and
+// doubled quotes.
+std::vector<std::string> split_csv(const std::string& line) {
+  std::vector<std::string> out;
+  std::string cur;
+  bool quoted = false;
+  for (size_t i = 0; i < line.size(); ++i) {
+    const char ch = line[i];
+    if (quoted) {
+      if (ch == '"' && i + 1 < line.size() && line[i + 1] == '"') { cur += '"'; ++i; }
+      else if (ch == '"') quoted = false;
+      else cur += ch;
+    } else if (ch == '"') {
+      quoted = true;
+    } else if (ch == ',') {
+      out.push_back(cur);
+      cur.clear();
+    } else if (ch != '\r') {
+      cur += ch;
+    }
+  }
+  out.push_back(cur);
+  return out;
+}
+
+}  // namespace
+
+bool read_population_csv(Config& cfg, std::string& err) {
+  std::ifstream f(cfg.csv_file);
+  if (!f) {
+    err = "Could not read population file: " + cfg.csv_file;
+    return false;
+  }
+  std::string line;
+  if (!std::getline(f, line)) {
+    cfg.number_of_agents = 0;
+    return true;
+  }
+  const std::vector<std::string> head = split_csv(line);
+  int col_ind = -1, col_age = -1, col_working = -1, col_pt = -1;
+  for (size_t k = 0; k < head.size(); ++k) {
+    if (head[k] == "ind") col_ind = static_cast<int>(k);
+    else if (head[k] == "age") col_age = static_cast<int>(k);
+    else if (head[k] == "working") col_working = static_cast<int>(k);
+    else if (head[k] == "pub_transport") col_pt = static_cast<int>(k);
+  }
+  const char* missing = col_ind < 0 ? "ind" : col_age < 0 ? "age" : col_working < 0 ? "working"
+                        : col_pt < 0 ? "pub_transport" : nullptr;
+  if (missing) {
+    err = std::string("Could not deserialize population line: missing field `") + missing + "`";
+    return false;
+  }
+  cfg.csv_working.clear();
+  cfg.csv_pub_transport.clear();
+  size_t row = 1;
+  auto boolean = [&](const std::string& v, uint8_t& out) {
+    if (v == "True") { out = 1; return true; }
+    if (v == "False") { out = 0; return true; }
+    err = "Could not deserialize population line " + std::to_string(row) + ": invalid value: string \"" + v +
+          "\", expected True or False";
+    return false;
+  };
+  while (std::getline(f, line)) {
+    ++row;
+    if (line.empty() || line == "\r") continue;
+    const std::vector<std::string> v = split_csv(line);
+    if (static_cast<int>(v.size()) <= std::max(std::max(col_ind, col_age), std::max(col_working, col_pt))) {
+      err = "Could not deserialize population line " + std::to_string(row) + ": too few fields";
+      return false;
+    }
+    const std::string& ind = v[col_ind];
+    if (ind.empty() || ind.find_first_not_of("0123456789") != std::string::npos || ind.size() > 10 ||
+        std::stoull(ind) > 0xFFFFFFFFull) {
+      err = "Could not deserialize population line " + std::to_string(row) + ": invalid `ind` \"" + ind + "\"";
+      return false;
+    }
+    uint8_t w = 0, pt = 0;
+    if (!boolean(v[col_working], w) || !boolean(v[col_pt], pt)) return false;
+    cfg.csv_working.push_back(w);
+    cfg.csv_pub_transport.push_back(pt);
+  }
+  cfg.number_of_agents = static_cast<uint32_t>(cfg.csv_working.size());
   return true;
 }
 

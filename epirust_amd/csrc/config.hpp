@@ -13,9 +13,12 @@ struct VaccinateConfig {
 };
 
 struct Config {
-  // population: Auto (common/src/config/population.rs:34-42)
+  // population: Auto (common/src/config/population.rs:34-42) or Csv (:26-31)
   uint32_t number_of_agents = 0;
   double public_transport_percentage = 0, working_percentage = 0;
+  bool csv_population = false;
+  std::string csv_file;
+  std::vector<uint8_t> csv_working, csv_pub_transport;  // per record, filled by read_population_csv
   // disease (common/src/disease/mod.rs:26-44)
   uint32_t regular_transmission_start_day = 0, high_transmission_start_day = 0, last_day = 0;
   uint32_t asymptomatic_last_day = 0, mild_infected_last_day = 0;
@@ -43,5 +46,10 @@ struct Config {
 
 // Parses `text`; on failure returns false and writes a serde-style message to `err`.
 bool parse_config(const std::string& text, Config& out, std::string& err);
+
+// Reads the Population::Csv file named by the config (grid.rs:194-231,
+// citizen/population_record.rs:23-43): records with ind (u32), age, working and
+// pub_transport ("True" / "False"). Sets number_of_agents to the record count.
+bool read_population_csv(Config& cfg, std::string& err);
 
 }  // namespace epi

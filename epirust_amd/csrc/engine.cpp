@@ -313,6 +313,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   for (uint32_t i = 0; i < s->n_sims; ++i) {
     std::string err;
     if (!parse_config(texts[i], s->cfg[i], err)) return fail(EPI_ERR_CONFIG, err);
+    if (s->cfg[i].csv_population && !read_population_csv(s->cfg[i], err)) return fail(EPI_ERR_PANIC, err);
     int rc = validate(s->cfg[i]);
     if (rc != EPI_OK) return rc;
   }

@@ -151,6 +151,11 @@ bool synthesize_population(const Config& cfg, uint64_t seed, Population& pop, st
           " agents available";
     return false;
   }
+  if (cfg.csv_population && N > 4u * H) {  // grid.rs:212-219
+    err = "Cannot accommodate citizens into homes! There are " + std::to_string(N) + " citizens, but " +
+          std::to_string(4u * H) + " home points";
+    return false;
+  }
   if (H > 0 && N > 4u * H) {
     err = "There are " + std::to_string((N + H - 1) / H) +
           " agents assigned to a house, but house capacity is 4";
@@ -172,8 +177,11 @@ bool synthesize_population(const Config& cfg, uint64_t seed, Population& pop, st
       const u32x4 a = draw(seed, i, kInitHour, SITE_INIT_A, 0);
       const u32x4 s = draw(seed, i, kInitHour, SITE_INIT_B, 0);
       const u32x4 c = draw(seed, i, kInitHour, SITE_INIT_C, 0);
-      const bool working = bernoulli(thr_work, join64(a.w[0], a.w[1]));
-      pt_draw[i] = working && bernoulli(thr_pt, join64(a.w[2], a.w[3]));
+      // Population::Csv takes working / public transport from the record
+      // (Citizen::from_record, citizen/mod.rs:155-180); Auto draws them.
+      const bool working = cfg.csv_population ? cfg.csv_working[i] != 0 : bernoulli(thr_work, join64(a.w[0], a.w[1]));
+      pt_draw[i] = cfg.csv_population ? cfg.csv_pub_transport[i] != 0
+                                      : working && bernoulli(thr_pt, join64(a.w[2], a.w[3]));
       uint32_t ws;
       if (!working) ws = WS_NA;
       else if (bernoulli(thr_staff, join64(s.w[0], s.w[1]))) ws = WS_STAFF;
@@ -184,10 +192,11 @@ bool synthesize_population(const Config& cfg, uint64_t seed, Population& pop, st
       if (total_si) keys[i] = SelKey{join64(c.w[2], c.w[3]), i};
     }
   });
-  // public transport users are capped in id order (citizen_factory.rs:72-83)
+  // public transport users are capped in id order (citizen_factory.rs:72-83); records of a
+  // Population::Csv keep their flag uncapped
   uint32_t users = 0;
   for (uint32_t i = 0; i < N; ++i)
-    if (pt_draw[i] && users < G.n_transport_points) {
+    if (pt_draw[i] && (cfg.csv_population || users < G.n_transport_points)) {
       pop.st[i] |= F_PT;
       ++users;
     }

@@ -705,7 +705,8 @@ epicpu_sim* epicpu_create(const epicpu_config* cfg, char* err, size_t errlen) {
 
     epicpu_sim* s = (epicpu_sim*)calloc(1, sizeof(epicpu_sim));
     s->cfg = *cfg;
-    s->n = cfg->number_of_agents;
+    s->n = cfg->csv_n > 0 ? cfg->csv_n : cfg->number_of_agents;
+    s->cfg.number_of_agents = s->n;
     s->g = cfg->grid_size;
     define_geography(s->g, &s->housing, &s->transport, &s->work, &s->hospital);
     s->houses = make_buildings(s->housing, 2);   /* HOME_SIZE */
@@ -734,6 +735,13 @@ epicpu_sim* epicpu_create(const epicpu_config* cfg, char* err, size_t errlen) {
                         cfg->si_infected_severe;
     if (total_si > s->n) {
         snprintf(msg, sizeof msg, "There are %u people set to infect, but only %u agents available", total_si, s->n);
+        set_err(err, errlen, msg);
+        free(s);
+        return NULL;
+    }
+    if (cfg->csv_n > 0 && s->n > 4u * H) { /* grid.rs:212-219 */
+        snprintf(msg, sizeof msg, "Cannot accommodate citizens into homes! There are %u citizens, but %u home points",
+                 s->n, 4u * H);
         set_err(err, errlen, msg);
         free(s);
         return NULL;
@@ -771,11 +779,17 @@ epicpu_sim* epicpu_create(const epicpu_config* cfg, char* err, size_t errlen) {
         draw(cfg->seed, i, INIT_HOUR, SITE_INIT_B, 0, b);
         citizen* c = &s->ag[i];
         c->id = i;
-        int working = bernoulli(thr_work, u64_of(a[0], a[1]));
         rect home = building_at(s->houses, i % H);
         rect office = building_at(s->offices, i % O);
-        int pt = bernoulli(thr_pt, u64_of(a[2], a[3])) && working && pt_users < s->n_tpoints;
-        if (pt) pt_users++;
+        int working, pt;
+        if (cfg->csv_n > 0) { /* Citizen::from_record (citizen/mod.rs:155-180): flags from the record */
+            working = cfg->csv_working[i] != 0;
+            pt = cfg->csv_pub_transport[i] != 0;
+        } else {
+            working = bernoulli(thr_work, u64_of(a[0], a[1]));
+            pt = bernoulli(thr_pt, u64_of(a[2], a[3])) && working && pt_users < s->n_tpoints;
+            if (pt) pt_users++;
+        }
         c->home = home;
         c->work = working ? office : home;
         c->uses_pt = (uint8_t)pt;

@@ -67,6 +67,11 @@ typedef struct {
     uint32_t vaccinate_at_hour[EPICPU_MAX_VACC];
     double vaccinate_percent[EPICPU_MAX_VACC];
     uint64_t seed;
+    /* Population::Csv (grid.rs:194-231): when csv_n > 0 the population is csv_n records
+       with these working / pub_transport flags (number_of_agents is ignored). */
+    uint32_t csv_n;
+    const uint8_t* csv_working;
+    const uint8_t* csv_pub_transport;
 } epicpu_config;
 
 /* Canonical per-agent dump, identical in layout to the product's epi_agents_soa. */

@@ -54,6 +54,9 @@ class EpicpuConfig(ctypes.Structure):
         ("vaccinate_at_hour", ctypes.c_uint32 * MAX_VACC),
         ("vaccinate_percent", ctypes.c_double * MAX_VACC),
         ("seed", ctypes.c_uint64),
+        ("csv_n", ctypes.c_uint32),
+        ("csv_working", ctypes.c_void_p),
+        ("csv_pub_transport", ctypes.c_void_p),
     ]
 
 
@@ -132,12 +135,18 @@ def config_from_json(cfg, seed=None):
         cfg = json.loads(cfg)
     c = EpicpuConfig()
     pop = cfg["population"]
-    if "Auto" not in pop:
-        raise ValueError("only Population::Auto is supported by the oracle")
-    auto = pop["Auto"]
-    c.number_of_agents = auto["number_of_agents"]
-    c.public_transport_percentage = auto["public_transport_percentage"]
-    c.working_percentage = auto["working_percentage"]
+    if "Csv" in pop:
+        working, pub = read_population_csv(pop["Csv"]["file"])
+        c._csv = (working, pub)  # keep the arrays alive with the struct
+        c.csv_n = len(working)
+        c.csv_working = working.ctypes.data
+        c.csv_pub_transport = pub.ctypes.data
+        c.number_of_agents = len(working)
+    else:
+        auto = pop["Auto"]
+        c.number_of_agents = auto["number_of_agents"]
+        c.public_transport_percentage = auto["public_transport_percentage"]
+        c.working_percentage = auto["working_percentage"]
     d = cfg["disease"]
     for k in ("regular_transmission_start_day", "high_transmission_start_day", "last_day", "asymptomatic_last_day",
               "mild_infected_last_day", "regular_transmission_rate", "high_transmission_rate", "death_rate",
@@ -171,6 +180,27 @@ def config_from_json(cfg, seed=None):
     c.n_vaccinate = nv
     c.seed = int(seed if seed is not None else cfg.get("seed", 0)) & 0xFFFFFFFFFFFFFFFF
     return c
+
+
+def read_population_csv(path):
+    """Population::Csv records (engine/src/citizen/population_record.rs:23-43) read with
+    Python's csv module: columns ind, age, working, pub_transport; flags "True"/"False"."""
+    import csv
+    working, pub = [], []
+    with open(path, newline="") as f:
+        for rec in csv.DictReader(f):
+            for k in ("ind", "age", "working", "pub_transport"):
+                if k not in rec:
+                    raise ValueError("missing field `%s`" % k)
+            int(rec["ind"])
+            flags = []
+            for k in ("working", "pub_transport"):
+                if rec[k] not in ("True", "False"):
+                    raise ValueError("expected True or False, got %r" % rec[k])
+                flags.append(rec[k] == "True")
+            working.append(flags[0])
+            pub.append(flags[1])
+    return np.array(working, dtype=np.uint8), np.array(pub, dtype=np.uint8)
 
 
 def alloc_agents(n):

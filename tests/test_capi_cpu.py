@@ -51,9 +51,12 @@ def test_config_errors_are_reported_not_aborted(cfg1):
     rc, msg, _ = _create(json.dumps(c))
     assert rc == 1 and "expected u32" in msg
     c = json.loads(json.dumps(cfg1))
-    c["population"] = {"Csv": {"file": "x.csv", "cols": []}}
+    c["population"] = {"Csv": {"file": "/nonexistent/x.csv", "cols": []}}
     rc, msg, _ = _create(json.dumps(c))
-    assert rc == 1 and "Csv" in msg
+    assert rc == 2 and "Could not read population file" in msg
+    c["population"] = {"Csv": {"file": "x.csv"}}
+    rc, msg, _ = _create(json.dumps(c))
+    assert rc == 1 and msg == "missing field `cols`"
     c = json.loads(json.dumps(cfg1))
     c["interventions"] = [{"Quarantine": {}}]
     rc, msg, _ = _create(json.dumps(c))
@@ -89,3 +92,18 @@ def test_reference_fixture_config_parses(cfg1):
     text = open(os.path.join(ROOT, "tests", "golden", "reference", "auto_pop.json")).read()
     rc, msg, _ = _create(text)
     assert rc in (0, 3), msg
+
+
+def test_csv_population_records_are_validated(cfg1, tmp_path):
+    p = tmp_path / "bad.csv"
+    p.write_text("ind,age,working,pub_transport\n0,30,True,False\n1,31,yes,False\n")
+    c = json.loads(json.dumps(cfg1))
+    c["population"] = {"Csv": {"file": str(p), "cols": ["age", "working", "pub_transport"]}}
+    rc, msg, _ = _create(json.dumps(c))
+    assert rc == 2 and "expected True or False" in msg and "line 3" in msg
+    p.write_text("ind,working,pub_transport\n0,True,False\n")
+    rc, msg, _ = _create(json.dumps(c))
+    assert rc == 2 and "missing field `age`" in msg
+    p.write_text("ind,age,working,pub_transport\n0,30,True,False\n")
+    rc, msg, _ = _create(json.dumps(c))
+    assert rc in (0, 3), msg  # parses; only the device check is left without a GPU

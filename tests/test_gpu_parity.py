@@ -200,3 +200,13 @@ def test_cfg4_batched_replicates_match_their_own_runs():
         assert (np.diff(r[:, 5].astype(np.int64) + r[:, 6]) >= 0).all()  # R + D never shrink
         assert (np.diff(r[:, 0].astype(np.int64)) == 1).all() and r[0, 0] == 1
         assert stopped[i]
+
+
+def test_csv_population_bit_exact(cfg1, tmp_path):
+    """Population::Csv read by the C++ host and by the oracle's Python reader."""
+    from tests.test_csv_population import csv_config, write_population
+    path = write_population(str(tmp_path / "pop.csv"), 8000)
+    cfg = csv_config(cfg1, path)
+    cfg["disease"]["regular_transmission_rate"] = 0.2
+    cfg["disease"]["high_transmission_rate"] = 0.5
+    run_lockstep(cfg, 400, "csv population", every=8)
