@@ -71,12 +71,25 @@ def oracle_rate(cfgs, target_seconds, seeds=None, threads=None):
         threads, "/".join(str(h) for h in hours_by[:4]) + ("/..." if len(hours_by) > 4 else ""), len(cfgs), spent)
 
 
+_hip = None
+
+
+def device_sync():
+    """Device-wide synchronisation (hipDeviceSynchronize) on both sides of a timed region."""
+    global _hip
+    import ctypes
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+    rc = _hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError("hipDeviceSynchronize failed: %d" % rc)
+
+
 def timed_steps(handle, hours):
-    import torch
-    torch.cuda.synchronize()
+    device_sync()
     t0 = time.perf_counter()
     rows, stopped = handle.step(hours)
-    torch.cuda.synchronize()
+    device_sync()
     return time.perf_counter() - t0, rows
 
 
@@ -98,8 +111,7 @@ def bench_headline(args, G):
         w.step(min(args.warmup, run_hours))
         w.close()
     sims = [make() for _ in range(max(1, math.ceil(args.steps / run_hours)))]
-    import torch
-    torch.cuda.synchronize()
+    device_sync()
     t0 = time.perf_counter()
     hours_done, remaining = 0, args.steps
     for s in sims:
@@ -109,7 +121,7 @@ def bench_headline(args, G):
         remaining -= k
         if remaining <= 0:
             break
-    torch.cuda.synchronize()
+    device_sync()
     elapsed = time.perf_counter() - t0
     for s in sims:
         s.close()

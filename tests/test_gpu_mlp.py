@@ -99,9 +99,10 @@ def test_training_reduces_loss_on_a_smooth_target():
     losses = []
     for step in range(400):
         idx = rng.integers(0, len(X), 256)
+        xb = np.ascontiguousarray(X[idx])  # named: the arrays must outlive the call
+        tb = np.ascontiguousarray(T[idx])
         loss = ctypes.c_float()
-        rc = G.lib().epi_mlp_train_step(h, np.ascontiguousarray(X[idx]).ctypes.data,
-                                        np.ascontiguousarray(T[idx]).ctypes.data, 256, 1e-3, ctypes.byref(loss))
+        rc = G.lib().epi_mlp_train_step(h, xb.ctypes.data, tb.ctypes.data, 256, 1e-3, ctypes.byref(loss))
         assert rc == 0
         losses.append(loss.value)
     assert np.mean(losses[-20:]) < 0.1 * np.mean(losses[:20])
