@@ -93,9 +93,9 @@ struct Agent {
   uint32_t id, h, w, tm, wsa;
   int32_t x, y;                     // current cell
   int32_t tx, ty;                   // chosen cell
-  bool have_mv = false, have_dz = false;
-  int inf_block = -1;
-  u32x4 mv, dz, inf;
+  // The hour's movement and disease draw blocks, computed once up front (one inlined
+  // Philox each) for the hours / states that can use them.
+  u32x4 mv{{0, 0, 0, 0}}, dz{{0, 0, 0, 0}};
   // 5x5 grid window around (wx, wy), one 64-bit row per grid row: bytes of columns
   // c0 .. c0+7 where c0 = (x + kPadX - 2) & ~3 and woff = x + kPadX - c0 (2..5).
   bool have_win = false;
@@ -107,20 +107,8 @@ struct Agent {
   __device__ __forceinline__ u32x4 dr(uint32_t site, uint32_t block) const {
     return philox4x32_10(id, h, site, block, D.seed_lo, D.seed_hi);
   }
-  __device__ __forceinline__ const u32x4& move_words() {
-    if (!have_mv) { mv = dr(SITE_MOVE, 0); have_mv = true; }
-    return mv;
-  }
-  __device__ __forceinline__ const u32x4& disease_words() {
-    if (!have_dz) { dz = dr(SITE_DISEASE, 0); have_dz = true; }
-    return dz;
-  }
-  __device__ __forceinline__ uint64_t infect_word(int k) {
-    const int b = k >> 1;
-    if (inf_block != b) { inf = dr(SITE_INFECT, static_cast<uint32_t>(b)); inf_block = b; }
-    const int i = 2 * (k & 1);
-    return join64(inf.w[i], inf.w[i + 1]);
-  }
+  __device__ __forceinline__ const u32x4& move_words() const { return mv; }
+  __device__ __forceinline__ const u32x4& disease_words() const { return dz; }
 
   __device__ __forceinline__ uint32_t state() const { return fld(w, B_STATE, M_STATE); }
   __device__ __forceinline__ uint32_t sev() const { return fld(w, B_SEV, M_SEV); }
@@ -251,30 +239,45 @@ struct Agent {
   }
 
   // default_disease_handler.rs:64-86 (OR over fixed per-neighbour draws)
-  // The 3x3 scan around (tx, ty) = (wx + ox, wy + OY) from window rows OY+1..OY+3.
-  template <int OY>
-  __device__ __forceinline__ bool scan_infectious(const Rect& a, int32_t ox) {
-    bool hit = false;
-    for_moore([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      if (hit) return;
-      const int32_t cx = tx + moore_dx(k), cy = ty + moore_dy(k);
-      if (!contains(a, cx, cy) || !in_grid(cx, cy)) return;
-      const uint32_t b = wbyte<OY + moore_dy(k)>(ox + moore_dx(k));
-      if (b < 2) return;
-      const uint64_t thr = b == 2 ? D.thr_regular : D.thr_high;
-      hit = bernoulli(thr, infect_word(k));
-    });
-    return hit;
-  }
+  // default_disease_handler.rs:64-86 as an OR over fixed per-neighbour draws: first the
+  // mask of eligible infectious neighbours of (tx, ty) from the window (rows chosen with
+  // bit masks, so no variable indexing), then one Philox block per pair of neighbours,
+  // drawn in Moore order only for eligible ones.
   __device__ __forceinline__ bool on_susceptible() {
     if ((w & F_WQ) || (w & F_VACC)) return false;
     const Rect a = current_area();
     cover(tx, ty);
-    const int32_t ox = tx - wx, oy = ty - wy;
-    if (oy < 0) return scan_infectious<-1>(a, ox);
-    if (oy > 0) return scan_infectious<1>(a, ox);
-    return scan_infectious<0>(a, ox);
+    const int32_t ox = tx - wx, oy = ty - wy;  // both in [-1, 1]
+    const uint64_t mm = oy < 0 ? ~0ull : 0ull, m0 = oy == 0 ? ~0ull : 0ull, mp = oy > 0 ? ~0ull : 0ull;
+    const uint64_t top = (r0 & mm) | (r1 & m0) | (r2 & mp);
+    const uint64_t mid = (r1 & mm) | (r2 & m0) | (r3 & mp);
+    const uint64_t bot = (r2 & mm) | (r3 & m0) | (r4 & mp);
+    uint32_t hot = 0, high = 0;
+    for_moore([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      const int32_t cx = tx + moore_dx(k), cy = ty + moore_dy(k);
+      const uint64_t r = moore_dy(k) < 0 ? top : (moore_dy(k) == 0 ? mid : bot);
+      const uint32_t b = static_cast<uint32_t>(r >> (8 * (woff + ox + moore_dx(k)))) & 0xFFu;
+      if (contains(a, cx, cy) && in_grid(cx, cy) && b >= 2) {
+        hot |= 1u << k;
+        if (b == 3) high |= 1u << k;
+      }
+    });
+    int blk = -1;
+    uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+    while (hot) {
+      const int k = __builtin_ctz(hot);
+      hot &= hot - 1;
+      if ((k >> 1) != blk) {
+        blk = k >> 1;
+        const u32x4 q = dr(SITE_INFECT, static_cast<uint32_t>(blk));
+        q0 = q.w[0]; q1 = q.w[1]; q2 = q.w[2]; q3 = q.w[3];
+      }
+      const uint64_t v = (k & 1) ? join64(q2, q3) : join64(q0, q1);
+      const uint64_t thr = ((high >> k) & 1u) ? D.thr_high : D.thr_regular;
+      if (bernoulli(thr, v)) return true;
+    }
+    return false;
   }
 
   // disease_state_machine.rs:53-70 + default_disease_handler.rs:41-62
@@ -464,6 +467,10 @@ __global__ void k_step(DevView v) {
   if (!S->active) return;  // uniform over the block
   const uint32_t h = S->hour + 1;
   const uint32_t hd = h % 24u;
+  // Hours of day 2..6 change nothing: hour 1 already set every non-staff current area to
+  // home, and no movement or disease update runs while asleep (citizen/mod.rs:244-248).
+  // k_epilogue repeats the previous hour's counts for them.
+  if (hd >= 2 && hd <= 6) return;
   const uint32_t local = first_lane(bm.first) + threadIdx.x;
   const bool valid = local < D.n;
   const uint8_t* occ = v.occ + D.cell_off;
@@ -487,6 +494,12 @@ __global__ void k_step(DevView v) {
     const uint32_t wsa0 = a.wsa;
     bool tm_dirty = false;
 
+    {
+      const uint32_t s0 = a.state(), v0 = a.sev();
+      if (hd == 0 || hd >= 7) a.mv = a.dr(SITE_MOVE, 0);
+      if (hd >= 7 && (s0 == ST_E || (s0 == ST_I && (v0 == SEV_PRE || (hd == 23 && v0 == SEV_SEVERE)))))
+        a.dz = a.dr(SITE_DISEASE, 0);
+    }
     if (hd == 0) {
       if (a.state() == ST_I) {  // increment_infection_day (state.rs:69-73)
         const uint32_t day = fld(a.w, B_DAY, M_DAY);
@@ -545,6 +558,15 @@ __global__ void k_epilogue(DevView v) {
     return;
   }
   __shared__ uint32_t s_sum[6];
+  const uint32_t h = S->hour + 1;
+  if (h % 24u >= 2 && h % 24u <= 6) {  // k_step skipped this hour: its counts equal the last row's
+    if (threadIdx.x == 0) {
+      const Counts7 prev = v.rows[D.row_off + (h - 2)];
+      const uint32_t c[6] = {prev.s, prev.e, prev.i, prev.h, prev.r, prev.d};
+      hour_epilogue(v, D, S, h, c);
+    }
+    return;
+  }
   if (threadIdx.x < 6) s_sum[threadIdx.x] = 0;
   __syncthreads();
   uint32_t acc[6] = {0, 0, 0, 0, 0, 0};
@@ -560,7 +582,7 @@ __global__ void k_epilogue(DevView v) {
     if ((threadIdx.x & (kWave - 1)) == 0 && x) atomicAdd(&s_sum[k], x);
   }
   __syncthreads();
-  if (threadIdx.x == 0) hour_epilogue(v, D, S, S->hour + 1, s_sum);
+  if (threadIdx.x == 0) hour_epilogue(v, D, S, h, s_sum);
 }
 
 __global__ void k_commit(DevView v) {
@@ -571,6 +593,8 @@ __global__ void k_commit(DevView v) {
   if (!S->commit_pending) return;
   const uint32_t h = S->hour;
   const uint32_t sweep = S->sweep;
+  // No agent moves or changes its grid byte at hours of day 1..6, so only a sweep needs work.
+  if (sweep == 0 && h % 24u >= 1 && h % 24u <= 6) return;
   const uint32_t local = first_lane(bm.first) + threadIdx.x;
   if (local < D.n) {
     const uint64_t gi = D.agent_off + local;
