@@ -24,6 +24,10 @@ EXE_PATH = os.path.join(PKG, "epirust-gpu")
 
 EPI_FLAG_DETERMINISTIC = 1
 EPI_FLAG_NO_GRAPH = 2
+EPI_FLAG_HOST_INIT = 4
+CITIZEN_STATE_DTYPE = np.dtype({"names": ["citizen_id", "x", "y", "state"],
+                                 "formats": [np.uint32, np.int32, np.int32, "S1"],
+                                 "offsets": [0, 4, 8, 12], "itemsize": 16})
 EVENT_NAMES = {0: ("vaccination", {}), 1: ("lockdown", {"status": "locked_down"}),
                2: ("lockdown", {"status": "lockdown_revoked"}), 3: ("build_new_hospital", {})}
 
@@ -78,6 +82,9 @@ SYMBOLS = {
     "epi_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_double), _P(ctypes.c_uint32),
                                    _P(ctypes.c_uint32)]),
     "epi_set_block_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "epi_citizen_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+                                          _P(ctypes.c_uint32), _P(ctypes.c_uint32)]),
+    "epi_stream_citizen_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "epi_last_error": (ctypes.c_char_p, []),
     "epi_destroy": (None, [ctypes.c_void_p]),
     "epi_cli_main": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_char_p)]),
@@ -203,6 +210,20 @@ class _Handle:
 
     def set_block_size(self, threads):
         _check(lib().epi_set_block_size(self._h, threads))
+
+    def citizen_states(self, index=0):
+        """(hour, records) after the last stepped hour: a structured array of
+        (citizen_id, x, y, state) in id order (CitizenStatesAtHr, citizen_state.rs:46-68)."""
+        n, hour = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().epi_citizen_states(self._h, index, None, 0, ctypes.byref(n), ctypes.byref(hour)))
+        out = np.zeros(max(n.value, 1), dtype=CITIZEN_STATE_DTYPE)
+        _check(lib().epi_citizen_states(self._h, index, out.ctypes.data, n.value, ctypes.byref(n),
+                                        ctypes.byref(hour)))
+        return hour.value, out[: n.value]
+
+    def stream_citizen_states(self, path):
+        """Open (path) or close (None) the JSON-lines citizen-state sink."""
+        _check(lib().epi_stream_citizen_states(self._h, None if path is None else os.fsencode(path)))
 
 
 class Simulation(_Handle):

@@ -28,7 +28,7 @@ HOST = ["-I" + os.path.join(ROCM, "include"), "-D__HIP_PLATFORM_AMD__"]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 HOST_SOURCES = ["config.cpp", "population.cpp", "engine.cpp", "cli.cpp", "mlp_engine.cpp"]
-DEVICE_SOURCES = ["kernels.hip", "mlp.hip"]
+DEVICE_SOURCES = ["kernels.hip", "init.hip", "stream.hip", "mlp.hip"]
 
 
 def _run(cmd):

@@ -21,9 +21,11 @@
 
 #include "epirust_gpu.h"
 #include "config.hpp"
+#include "init.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
 #include "population.hpp"
+#include "stream.hpp"
 
 using namespace epi;
 
@@ -87,6 +89,33 @@ struct DeviceBuffers {
   }
 };
 
+// File sink of the citizen-state stream: one JSON line per message of the reference's
+// citizen_states_updated topic (listeners/events_kafka_producer.rs:62-67, 90-121). The
+// hour's body is formatted on the device (stream.hip) and copied into one of two pinned
+// buffers, so the file write of hour h overlaps the step of hour h + 1.
+struct StateSink {
+  FILE* f = nullptr;
+  std::string path;
+  uint64_t cap = 0;  // text bytes of one hour, upper bound
+  char* d_text = nullptr;
+  unsigned long long* d_block_off = nullptr;
+  unsigned long long* d_total = nullptr;
+  char* h_text[2] = {nullptr, nullptr};
+  unsigned long long* h_total = nullptr;  // pinned
+  hipEvent_t ev_total = nullptr;
+  hipEvent_t ev_text[2] = {nullptr, nullptr};
+  ~StateSink() {
+    if (f) std::fclose(f);
+    for (hipEvent_t e : {ev_total, ev_text[0], ev_text[1]})
+      if (e) (void)hipEventDestroy(e);
+    for (char* p : h_text)
+      if (p) (void)hipHostFree(p);
+    if (h_total) (void)hipHostFree(h_total);
+    for (void* p : {static_cast<void*>(d_text), static_cast<void*>(d_block_off), static_cast<void*>(d_total)})
+      if (p) (void)hipFree(p);
+  }
+};
+
 }  // namespace
 
 struct epi_sim {
@@ -116,6 +145,7 @@ struct epi_sim {
   bool all_stopped = false;           // every simulation reported inactive
   std::vector<uint32_t> rows_returned;
   std::vector<SimState> state_h;
+  std::unique_ptr<StateSink> sink;  // citizen-state stream, single-simulation handles only
 
   ~epi_sim() {
     if (day_graph) (void)hipGraphExecDestroy(day_graph);
@@ -232,11 +262,69 @@ bool all_inactive(const epi_sim* s, const uint32_t* flags) {
   return true;
 }
 
+int write_state_line(StateSink& k, uint32_t hour, uint64_t bytes, const char* text) {
+  // CitizenStatesAtHr serialized by serde_json: {"hr":h,"citizen_states":[...]}
+  if (std::fprintf(k.f, "{\"hr\":%u,\"citizen_states\":[", hour) < 0 || std::fwrite(text, 1, bytes, k.f) != bytes ||
+      std::fputs("]}\n", k.f) < 0)
+    return fail(EPI_ERR_IO, "Failed to write citizen states to " + k.path);
+  return EPI_OK;
+}
+
+// The hour loop while the citizen-state stream is open: hour by hour, each hour's records
+// formatted on the device right after its commit. The file write of one hour runs while
+// the GPU steps the next. A zero-length body means the simulation stopped earlier.
+int advance_streaming(epi_sim* s, uint32_t target) {
+  StateSink& k = *s->sink;
+  const SimDesc& D = s->desc[0];
+  StreamArgs a{s->v.pos + D.agent_off, s->v.st + D.agent_off, D.n, s->v.state, 0u,
+               k.d_block_off, k.d_total, k.d_text};
+  uint32_t slot = 0, prev_hour = 0;
+  uint64_t prev_bytes = 0;
+  bool have_prev = false;
+  int rc = EPI_OK;
+  while (s->host_hour < target) {
+    const uint32_t h = s->host_hour + 1;
+    rc = enqueue_hour(s, h);
+    if (rc != EPI_OK) return rc;
+    a.hour = h;
+    HIP_TRY(launch_citizen_states(a, s->stream));
+    HIP_TRY(hipMemcpyAsync(k.h_total + slot, k.d_total, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipEventRecord(k.ev_total, s->stream));
+    ++s->host_hour;
+    if (have_prev) {  // overlaps the hour just enqueued
+      HIP_TRY(hipEventSynchronize(k.ev_text[slot ^ 1u]));
+      rc = write_state_line(k, prev_hour, prev_bytes, k.h_text[slot ^ 1u]);
+      if (rc != EPI_OK) return rc;
+      have_prev = false;
+    }
+    HIP_TRY(hipEventSynchronize(k.ev_total));
+    const uint64_t bytes = k.h_total[slot];
+    if (bytes == 0) break;
+    if (bytes > k.cap) return fail(EPI_ERR_DEVICE, "citizen-state text exceeds its bound");
+    HIP_TRY(hipMemcpyAsync(k.h_text[slot], k.d_text, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipEventRecord(k.ev_text[slot], s->stream));
+    prev_hour = h;
+    prev_bytes = bytes;
+    have_prev = true;
+    slot ^= 1u;
+  }
+  if (have_prev) {
+    HIP_TRY(hipEventSynchronize(k.ev_text[slot ^ 1u]));
+    rc = write_state_line(k, prev_hour, prev_bytes, k.h_text[slot ^ 1u]);
+    if (rc != EPI_OK) return rc;
+  }
+  if (std::fflush(k.f) != 0) return fail(EPI_ERR_IO, "Failed to write citizen states to " + k.path);
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return EPI_OK;
+}
+
 // Enqueue hours until `target` (or until every simulation has stopped). Whole days are
 // replayed from the captured graph; stop flags are polled one day behind the GPU.
 int advance(epi_sim* s, uint32_t target) {
   if (s->any_empty && target > s->host_hour) return fail(EPI_ERR_PANIC, "No citizens!");
   if (s->all_stopped) return EPI_OK;
+  if (s->sink) return advance_streaming(s, target);
   const bool graph_ok = !(s->flags & EPI_FLAG_NO_GRAPH);
   uint32_t day_index = 0;
   bool pending[2] = {false, false};
@@ -320,17 +408,22 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   int rc = check_device();
   if (rc != EPI_OK) return rc;
 
-  // population synthesis (host), several simulations at a time
+  // geometry and the reference's panic checks on the host; the agents themselves are
+  // synthesized on the device unless EPI_FLAG_HOST_INIT asks for the host restatement
+  const bool host_init = (flags & EPI_FLAG_HOST_INIT) != 0;
   std::vector<Population> pops(s->n_sims);
   std::vector<std::string> errs(s->n_sims);
   std::vector<char> ok(s->n_sims, 0);
   {
-    const unsigned nt = std::max(1u, std::min<unsigned>(s->n_sims, 8u));
+    const unsigned nt = host_init ? std::max(1u, std::min<unsigned>(s->n_sims, 8u)) : 1u;
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; ++t)
       th.emplace_back([&, t] {
         for (uint32_t i = t; i < s->n_sims; i += nt)
-          ok[i] = synthesize_population(s->cfg[i], s->seed[i], pops[i], errs[i]) ? 1 : 0;
+          ok[i] = (host_init ? synthesize_population(s->cfg[i], s->seed[i], pops[i], errs[i])
+                             : prepare_population(s->cfg[i], pops[i], errs[i]))
+                      ? 1
+                      : 0;
       });
     for (auto& x : th) x.join();
   }
@@ -468,6 +561,19 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     const SimDesc& D = s->desc[i];
     const Population& P = pops[i];
     const size_t nb = static_cast<size_t>(D.n) * sizeof(uint32_t);
+    SimState st;
+    std::memset(&st, 0, sizeof st);
+    st.active = (s->cfg[i].hours > 1 && D.n > 0) ? 1u : 0u;
+    st.first_vacant = ~0u;
+    s->state_h[i] = st;
+    if (!host_init) {
+      const InitTargets t{v.pos + D.agent_off, v.st + D.agent_off, v.tm + D.agent_off, v.wsa + D.agent_off,
+                          v.occ + D.cell_off, D.pitch, padded_cells(P.geo.stride, P.geo.height)};
+      std::string err;
+      const hipError_t e = synthesize_on_device(s->cfg[i], s->seed[i], P.geo, t, s->stream, err);
+      if (e != hipSuccess) return fail(EPI_ERR_DEVICE, "device population synthesis (" + err + "): " + hipGetErrorString(e));
+      continue;
+    }
     if (D.n) {
       HIP_TRY(hipMemcpy(v.pos + D.agent_off, P.pos.data(), nb, hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(v.st + D.agent_off, P.st.data(), nb, hipMemcpyHostToDevice));
@@ -482,11 +588,6 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
           host_grid_byte(s->cfg[i], P.st[a]);
     }
     HIP_TRY(hipMemcpy(v.occ + D.cell_off, grid.data(), cells, hipMemcpyHostToDevice));
-    SimState st;
-    std::memset(&st, 0, sizeof st);
-    st.active = (s->cfg[i].hours > 1 && D.n > 0) ? 1u : 0u;
-    st.first_vacant = ~0u;
-    s->state_h[i] = st;
   }
   HIP_TRY(hipMemcpy(v.state, s->state_h.data(), s->n_sims * sizeof(SimState), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d_hosp_blocks, s->hosp_blocks.data(), s->hosp_blocks.size() * sizeof(BlockRef),
@@ -535,9 +636,112 @@ const char* event_data(uint32_t kind) {
   return "{}";
 }
 
+std::string area_json(const Rect& r) {
+  // Area {location_id, start_offset, end_offset} (geography/area.rs:30-35); location_id is
+  // the standalone engine id "0" (engine_app.rs:30)
+  char buf[160];
+  std::snprintf(buf, sizeof buf,
+                "{\"location_id\":\"0\",\"start_offset\":{\"x\":%d,\"y\":%d},\"end_offset\":{\"x\":%d,\"y\":%d}}",
+                r.x0, r.y0, r.x1, r.y1);
+  return buf;
+}
+
+// The Grid message published once before the first hour (grid_updated,
+// events_kafka_producer.rs:101-115): serde field order of Grid (geography/grid.rs:35-47),
+// houses and offices in area_factory order (geography/area.rs:95-117).
+int write_grid_line(StateSink& k, const Geometry& G, uint32_t hosp_version) {
+  std::string out;
+  out.reserve(96u * (G.n_houses + G.n_offices) + 1024u);
+  out += "{\"grid_size\":" + std::to_string(G.g);
+  out += ",\"housing_area\":" + area_json(G.housing);
+  out += ",\"work_area\":" + area_json(G.work);
+  out += ",\"transport_area\":" + area_json(G.transport);
+  out += ",\"hospital_area\":" + area_json(hosp_version ? G.hosp1 : G.hosp0);
+  out += ",\"houses\":[";
+  for (uint32_t h = 0; h < G.n_houses; ++h) {
+    if (h) out += ',';
+    out += area_json(building_rect(G.house_sx, G.house_sy, G.house_nx, 2, h));
+  }
+  out += "],\"offices\":[";
+  for (uint32_t o = 0; o < G.n_offices; ++o) {
+    if (o) out += ',';
+    out += area_json(building_rect(G.office_sx, G.office_sy, G.office_nx, 10, o));
+  }
+  out += "]}\n";
+  if (std::fwrite(out.data(), 1, out.size(), k.f) != out.size())
+    return fail(EPI_ERR_IO, "Failed to write citizen states to " + k.path);
+  return EPI_OK;
+}
+
+int open_sink(epi_sim* s, const std::string& path) {
+  if (s->n_sims != 1) return fail(EPI_ERR_ARG, "the citizen-state stream needs a single-simulation handle");
+  int rc = fetch_states(s);
+  if (rc != EPI_OK) return rc;
+  std::unique_ptr<StateSink> k(new StateSink());
+  k->path = path;
+  const SimDesc& D = s->desc[0];
+  k->cap = static_cast<uint64_t>(D.n) * max_record_bytes(D.n, D.stride, D.height);
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&k->d_text), std::max<uint64_t>(k->cap, 1)));
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&k->d_block_off),
+                    std::max<uint64_t>(stream_blocks(D.n), 1) * sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(reinterpret_cast<void**>(&k->d_total), sizeof(unsigned long long)));
+  for (char*& p : k->h_text) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<uint64_t>(k->cap, 1)));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&k->h_total), 2 * sizeof(unsigned long long)));
+  HIP_TRY(hipEventCreateWithFlags(&k->ev_total, hipEventDisableTiming));
+  for (hipEvent_t& e : k->ev_text) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  k->f = std::fopen(path.c_str(), "w");
+  if (!k->f) return fail(EPI_ERR_IO, "Failed to create citizen state file: " + path);
+  rc = write_grid_line(*k, s->geo[0], s->state_h[0].hosp_version);
+  if (rc != EPI_OK) return rc;
+  s->sink = std::move(k);
+  return EPI_OK;
+}
+
+// Closes the stream with the reference's end message (events_kafka_producer.rs:75-87).
+int close_sink(epi_sim* s) {
+  if (!s->sink) return EPI_OK;
+  StateSink& k = *s->sink;
+  const bool ok = std::fputs("{\"simulation_ended\": true}\n", k.f) >= 0 && std::fclose(k.f) == 0;
+  k.f = nullptr;
+  const std::string path = k.path;
+  s->sink.reset();
+  if (!ok) return fail(EPI_ERR_IO, "Failed to write citizen states to " + path);
+  return EPI_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int epi_stream_citizen_states(epi_sim* sim, const char* path) {
+  if (!sim) return fail(EPI_ERR_ARG, "sim is null");
+  int rc = close_sink(sim);
+  if (rc != EPI_OK || !path) return rc;
+  return open_sink(sim, path);
+}
+
+int epi_citizen_states(epi_sim* sim, uint32_t index, epi_citizen_state* out, uint32_t cap, uint32_t* n,
+                       uint32_t* hour) {
+  if (!sim || index >= sim->n_sims) return fail(EPI_ERR_ARG, "bad handle or index");
+  int rc = fetch_states(sim);
+  if (rc != EPI_OK) return rc;
+  const SimDesc& D = sim->desc[index];
+  if (n) *n = D.n;
+  if (hour) *hour = sim->state_h[index].hour;
+  const uint32_t k = std::min(cap, D.n);
+  if (!out || k == 0) return EPI_OK;
+  std::vector<uint32_t> pos(k), st(k);
+  HIP_TRY(hipMemcpy(pos.data(), sim->v.pos + D.agent_off, k * 4ull, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(st.data(), sim->v.st + D.agent_off, k * 4ull, hipMemcpyDeviceToHost));
+  static const char letters[5] = {'s', 'e', 'i', 'r', 'd'};
+  for (uint32_t i = 0; i < k; ++i) {
+    out[i].citizen_id = i;
+    out[i].x = static_cast<int32_t>(pos[i] & 0xFFFFu);
+    out[i].y = static_cast<int32_t>(pos[i] >> 16);
+    out[i].state = letters[std::min<uint32_t>(st_state(st[i]), 4u)];
+  }
+  return EPI_OK;
+}
 
 const char* epi_last_error(void) { return g_last_error.c_str(); }
 
@@ -689,6 +893,11 @@ int epi_run(epi_sim* sim, const char* output_dir, char* final_path, size_t cap) 
   std::string out_dir = dir + (dir.empty() || dir.back() == '/' ? "" : "/") + "output";
   (void)::mkdir(out_dir.c_str(), 0777);
   const std::string base = out_dir + "/simulation_0_" + utc_stamp();
+  // citizen-state messages go to a JSON-lines file in place of the Kafka topic
+  if (sim->cfg[0].enable_citizen_state_messages) {
+    int rc = epi_stream_citizen_states(sim, listener_path(base + "_citizen_states.jsonl").c_str());
+    if (rc != EPI_OK) return rc;
+  }
   std::vector<epi_counts> rows;
   const uint32_t chunk = 24u * 50u;
   std::vector<epi_counts> buf(chunk);
@@ -699,6 +908,10 @@ int epi_run(epi_sim* sim, const char* output_dir, char* final_path, size_t cap) 
     if (rc != EPI_OK) return rc;
     rows.insert(rows.end(), buf.begin(), buf.begin() + nw);
     if (stopped || nw == 0) break;
+  }
+  if (sim->sink) {
+    int rc = close_sink(sim);
+    if (rc != EPI_OK) return rc;
   }
   // CSV (listeners/csv_service.rs:44-71): serde field order of Counts, one row per hour
   const std::string csv_path = listener_path(base + ".csv");

@@ -136,7 +136,7 @@ uint32_t choose2(int32_t lo, uint32_t m, const uint32_t* w, int32_t* out) {
 
 }  // namespace
 
-bool synthesize_population(const Config& cfg, uint64_t seed, Population& pop, std::string& err) {
+bool prepare_population(const Config& cfg, Population& pop, std::string& err) {
   if (!build_geometry(cfg, pop.geo, err)) return false;
   const Geometry& G = pop.geo;
   const uint32_t N = cfg.number_of_agents;
@@ -161,6 +161,19 @@ bool synthesize_population(const Config& cfg, uint64_t seed, Population& pop, st
           " agents assigned to a house, but house capacity is 4";
     return false;
   }
+  pop.counts0[0] = 0;
+  pop.counts0[1] = N - total_si;
+  pop.counts0[2] = cfg.si_exposed;
+  pop.counts0[3] = total_si - cfg.si_exposed;
+  return true;
+}
+
+bool synthesize_population(const Config& cfg, uint64_t seed, Population& pop, std::string& err) {
+  if (!prepare_population(cfg, pop, err)) return false;
+  const Geometry& G = pop.geo;
+  const uint32_t N = cfg.number_of_agents;
+  const uint32_t H = G.n_houses;
+  const uint32_t total_si = cfg.si_exposed + cfg.si_asymptomatic + cfg.si_mild + cfg.si_severe;
   pop.pos.assign(N, 0);
   pop.st.assign(N, 0);
   pop.tm.assign(N, 0);
@@ -233,10 +246,6 @@ bool synthesize_population(const Config& cfg, uint64_t seed, Population& pop, st
         for (uint32_t c = 0; c < my && j < k; ++c, ++j) pop.pos[h + j * H] = pack_pos(xs[a], ys[c]);
     }
   });
-  pop.counts0[0] = 0;
-  pop.counts0[1] = N - total_si;
-  pop.counts0[2] = cfg.si_exposed;
-  pop.counts0[3] = total_si - cfg.si_exposed;
   return true;
 }
 

@@ -30,8 +30,12 @@ uint64_t bernoulli_threshold(double p);
 // Geography (geography/mod.rs:33-70, grid.rs:240-261) for `agents` on grid `g`.
 bool build_geometry(const Config& cfg, Geometry& geo, std::string& err);
 
-// Deterministic-mode population synthesis; returns false with the reference's panic
-// text on the conditions the reference panics on.
+// Geometry, the reference's panic checks (house capacity, starting infections > N,
+// Population::Csv home points) and the hour-0 counts; returns false with the panic text.
+bool prepare_population(const Config& cfg, Population& pop, std::string& err);
+
+// Deterministic-mode population synthesis on the host: prepare_population, then the
+// agents' SoA words (the device path is synthesize_on_device in init.hpp).
 bool synthesize_population(const Config& cfg, uint64_t seed, Population& pop, std::string& err);
 
 // Bit helpers shared with the runtime's read-back.

@@ -1,0 +1,114 @@
+"""Summaries of rocprofv3 output for the hour-step kernels.
+
+    python -m epirust_amd.prof_summary trace <kernel_stats.csv> [--command "..."]
+    python -m epirust_amd.prof_summary traffic <fetch_dir> <write_dir> --agents N --grid G [--command "..."]
+
+`trace` turns a `--kernel-trace --stats` kernel_stats.csv into a markdown table (calls,
+average / min / max duration per kernel). `traffic` reads two `--pmc FETCH_SIZE` and
+`--pmc WRITE_SIZE` counter_collection.csv files of `bench.py --profile-hours H` (one
+dispatch per kernel and hour, then a 64 MiB device-to-device copy). It scales each
+kernel's KiB by the ratio the copy dispatch needs to read/write exactly 64 MiB, reports
+HBM bytes per simulated hour, and prints the JSON that bench.py reads as roofline.traffic.
+"""
+import argparse
+import collections
+import csv
+import json
+import os
+import sys
+
+SHORT = {"k_step": "k_step", "k_commit": "k_commit", "k_epilogue": "k_epilogue",
+         "k_hospital_scan": "k_hospital_scan", "copyBuffer": "copy (calibration)", "k_fill_u64": "k_fill_u64"}
+
+
+def short(name):
+    for k, v in SHORT.items():
+        if k in name:
+            return v
+    return name[:40]
+
+
+def trace(path, command):
+    rows = list(csv.DictReader(open(path)))
+    out = ["# Kernel trace summary", "", "Command: `%s`" % command, "",
+           "| kernel | calls | average µs | min µs | max µs | total ms | share |", "|---|---|---|---|---|---|---|"]
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        out.append("| %s | %s | %.2f | %.2f | %.2f | %.3f | %.1f%% |" % (
+            short(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
+            float(r["MaxNs"]) / 1e3, float(r["TotalDurationNs"]) / 1e6, 100.0 * float(r["TotalDurationNs"]) / total))
+    return "\n".join(out) + "\n"
+
+
+def _counter(path, counter):
+    per = collections.defaultdict(float)
+    names = {}
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        d = int(r["Dispatch_Id"])
+        per[d] += float(r["Counter_Value"])
+        names[d] = r["Kernel_Name"]
+    return per, names
+
+
+def traffic(fetch_csv, write_csv, agents, grid, hours, command):
+    fetch, names = _counter(fetch_csv, "FETCH_SIZE")
+    write, names_w = _counter(write_csv, "WRITE_SIZE")
+    copies = [d for d in fetch if "copyBuffer" in names[d]]
+    cal = max(copies)  # the calibration copy is the last copy dispatch
+    kib = 64 * 1024.0
+    f_scale = kib / fetch[cal]
+    copies_w = [d for d in write if "copyBuffer" in names_w[d]]
+    w_scale = kib / write[max(copies_w)]
+    per_kernel = collections.defaultdict(lambda: [0.0, 0.0, 0])
+    for d, v in fetch.items():
+        k = short(names[d])
+        if k.startswith("k_") and k != "k_fill_u64":
+            per_kernel[k][0] += v * f_scale * 1024
+            per_kernel[k][2] += 1
+    for d, v in write.items():
+        k = short(names_w[d])
+        if k.startswith("k_") and k != "k_fill_u64":
+            per_kernel[k][1] += v * w_scale * 1024
+    hour_bytes = sum(f + w for f, w, _ in per_kernel.values()) / hours
+    algo = 32.0 * agents + 8.0 * grid * grid
+    lines = ["# HBM traffic per simulated hour", "", "Commands:", ""] + ["    " + c for c in command.split(" && ")] + [
+        "", "Calibration: the 64 MiB device-to-device copy read %.0f KiB (FETCH_SIZE) and wrote %.0f KiB "
+            "(WRITE_SIZE); kernel figures are scaled by %.3f (fetch) and %.3f (write)." % (
+                fetch[cal], write[max(copies_w)], f_scale, w_scale), "",
+        "| kernel | dispatches | fetch MB / dispatch | write MB / dispatch |", "|---|---|---|---|"]
+    for k, (f, w, n) in sorted(per_kernel.items()):
+        lines.append("| %s | %d | %.2f | %.2f |" % (k, n, f / n / 1e6, w / n / 1e6))
+    lines += ["", "Over %d hours: %.1f MB of HBM traffic per simulated hour; the roofline model's algorithmic "
+                  "bytes are 32·N + 8·g² = %.1f MB per hour." % (hours, hour_bytes / 1e6, algo / 1e6)]
+    return "\n".join(lines) + "\n", {"hbm_bytes_per_hour": hour_bytes, "algorithmic_bytes_per_hour": algo,
+                                     "hours": hours, "fetch_scale": f_scale, "write_scale": w_scale}
+
+
+def main(argv):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=["trace", "traffic"])
+    ap.add_argument("paths", nargs="+")
+    ap.add_argument("--command", default="")
+    ap.add_argument("--agents", type=int, default=1000000)
+    ap.add_argument("--grid", type=int, default=2000)
+    ap.add_argument("--hours", type=int, default=48)
+    ap.add_argument("--out", default="")
+    ap.add_argument("--json", default="")
+    a = ap.parse_args(argv)
+    if a.mode == "trace":
+        text = trace(a.paths[0], a.command)
+    else:
+        text, js = traffic(a.paths[0], a.paths[1], a.agents, a.grid, a.hours, a.command)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(js, f, indent=1)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(text)
+    sys.stdout.write(text)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -31,6 +31,7 @@ extern "C" {
 /* flags for epi_create / epi_batch_create */
 #define EPI_FLAG_DETERMINISTIC 1u   /* Philox streams keyed (seed, agent id, hour); the only mode */
 #define EPI_FLAG_NO_GRAPH 2u        /* launch hour kernels directly instead of replaying day graphs */
+#define EPI_FLAG_HOST_INIT 4u       /* synthesize the population with the host restatement, not on the device */
 
 /* One CSV row of the epicurve; mirrors Counts (engine/src/models/events/counts.rs:25-34). */
 typedef struct {
@@ -92,6 +93,29 @@ int epi_run(epi_sim* sim, const char* output_dir, char* final_path, size_t cap);
 
 /* Current per-agent state of simulation `index` of the handle (0 for epi_create). */
 int epi_read_agents(epi_sim* sim, uint32_t index, epi_agents_soa* dst);
+
+/* One citizen-state record: CitizenState {citizen_id, state, location}
+ * (engine/src/models/events/citizen_state.rs:26-31); state is 's', 'e', 'i', 'r' or 'd'
+ * (:35-43) and citizen_id the dense deterministic id in place of the UUID. */
+typedef struct {
+  uint32_t citizen_id;
+  int32_t x, y;
+  char state;
+} epi_citizen_state;
+
+/* The records of simulation `index` after its last stepped hour, in id order: the
+ * CitizenStatesAtHr buffer (citizen_state.rs:46-68) filled by citizen_state_updated
+ * (engine/src/allocation_map.rs:122-124). *n = agents, *hour = last stepped hour. */
+int epi_citizen_states(epi_sim* sim, uint32_t index, epi_citizen_state* out, uint32_t cap, uint32_t* n,
+                       uint32_t* hour);
+
+/* Opens a file sink for the citizen-state stream in place of the Kafka topic
+ * citizen_states_updated (engine/src/listeners/events_kafka_producer.rs:62-121): one JSON
+ * line per message, the Grid first, then {"hr":h,"citizen_states":[...]} for every hour
+ * epi_step runs (formatted on the device). path NULL closes the sink after the
+ * {"simulation_ended": true} message. Single-simulation handles only. epi_run opens and
+ * closes it itself when the config sets enable_citizen_state_messages. */
+int epi_stream_citizen_states(epi_sim* sim, const char* path);
 
 /* Counts of the last completed hour (hour 0: counts_at_start, engine/src/utils/util.rs:45-51). */
 int epi_current_counts(epi_sim* sim, uint32_t index, epi_counts* out);
