@@ -175,6 +175,26 @@ def bench_interventions(G, cpu_seconds):
     return out
 
 
+def bench_init(G):
+    """epi_create time of the 10M-agent config (allocation + population synthesis + grid),
+    with the population synthesized by the device kernels and by the host restatement
+    (multithreaded C++), after one untimed device create."""
+    with open(os.path.join(CONFIGS, "cfg3_10m_interventions.json")) as f:
+        cfg = json.load(f)
+    out = {"agents": cfg["population"]["Auto"]["number_of_agents"]}
+    runs = [("warm", G.EPI_FLAG_DETERMINISTIC), ("device_s", G.EPI_FLAG_DETERMINISTIC),
+            ("host_s", G.EPI_FLAG_DETERMINISTIC | G.EPI_FLAG_HOST_INIT)]
+    for name, flags in runs:
+        t0 = time.perf_counter()
+        s = G.Simulation(cfg, flags=flags)
+        device_sync()
+        dt = time.perf_counter() - t0
+        s.close()
+        if name != "warm":
+            out[name] = dt
+    return out
+
+
 def profile_driver(args, G):
     import ctypes
     with open(args.config) as f:
@@ -264,6 +284,7 @@ def main():
     if not args.skip_extra:
         result["replicates"] = bench_replicates(G, cpu_s / 3)
         result["interventions_10m"] = bench_interventions(G, cpu_s / 3)
+        result["create_10m"] = bench_init(G)
     print(json.dumps(result))
 
 

@@ -61,7 +61,9 @@ enum {
     SITE_INIT_A = 16,   /* (w0,w1) working, (w2,w3) public transport */
     SITE_INIT_B = 17,   /* (w0,w1) hospital staff, w2 immunity */
     SITE_INIT_C = 18,   /* (w0,w1) essential worker, (w2,w3) starting-infection key */
-    SITE_INIT_HOUSE = 19 /* counter id = house index: w0,w1 x permutation, w2,w3 y permutation */
+    SITE_INIT_HOUSE = 19, /* counter id = house index: w0,w1 x permutation, w2,w3 y permutation */
+    SITE_REF_STREAM = 32, /* reference-order mode: the agent-hour's sequential stream, block = draw index */
+    SITE_REF_MERGE = 33   /* reference-order mode: counter id = hour; the merge permutation's stream */
 };
 
 static void draw(uint64_t seed, uint32_t id, uint32_t hour, uint32_t site, uint32_t block, uint32_t w[4]) {
@@ -248,7 +250,18 @@ struct epicpu_sim {
     event* events;
     uint32_t n_events, cap_events;
     uint32_t threads;
+    int ref_order;  /* epicpu_set_reference_order */
+    uint32_t* perm;
 };
+
+int epicpu_set_reference_order(epicpu_sim* s, int on) {
+    if (on && !s->perm) {
+        s->perm = (uint32_t*)malloc((size_t)(s->n ? s->n : 1) * sizeof(uint32_t));
+        if (!s->perm) return -1;
+    }
+    s->ref_order = on ? 1 : 0;
+    return 0;
+}
 
 void epicpu_set_threads(epicpu_sim* s, uint32_t threads) {
     s->threads = threads < 1 ? 1 : (threads > 64 ? 64 : threads);
@@ -275,13 +288,35 @@ static inline uint32_t immunity_day(const citizen* c) { /* citizen/mod.rs:184: (
     return (uint32_t)((int32_t)infection_day(c) + c->immunity);
 }
 
-/* Per-(agent, hour) random draws with fixed counters. */
+/* Per-(agent, hour) random draws with fixed counters; in reference-order mode a single
+   sequential stream consumed in the reference's call order instead. */
 typedef struct {
     const epicpu_sim* s;
     uint32_t id, hour;
     int have_move, have_disease, infect_block;
     uint32_t mv[4], dz[4], inf[4];
+    int ref;
+    uint32_t site, sbuf[4], sblock, spos;
 } draws;
+
+static uint32_t next_u32(draws* d) {
+    if (d->spos == 4) { draw(d->s->cfg.seed, d->id, d->hour, d->site, d->sblock++, d->sbuf); d->spos = 0; }
+    return d->sbuf[d->spos++];
+}
+/* rand's gen::<u64>() from a 32-bit source: low word first (rand_core next_u64_via_u32). */
+static uint64_t next_u64(draws* d) {
+    uint32_t lo = next_u32(d);
+    return u64_of(next_u32(d), lo);
+}
+/* rand 0.8.5 UniformInt<u32>::sample_single over [0, n), n >= 1: widening multiply with
+   rejection zone (n << leading_zeros(n)) - 1. */
+static uint32_t gen_range(draws* d, uint32_t n) {
+    const uint32_t zone = (n << __builtin_clz(n)) - 1u;
+    for (;;) {
+        uint64_t m = (uint64_t)next_u32(d) * n;
+        if ((uint32_t)m <= zone) return (uint32_t)(m >> 32);
+    }
+}
 
 static const uint32_t* move_words(draws* d) {
     if (!d->have_move) { draw(d->s->cfg.seed, d->id, d->hour, SITE_MOVE, 0, d->mv); d->have_move = 1; }
@@ -300,8 +335,13 @@ static uint64_t infect_u64(draws* d, int k) {
 
 /* area.rs:76-81: x then y, both inclusive. */
 static point random_point(draws* d, rect a) {
-    const uint32_t* w = move_words(d);
     point p;
+    if (d->ref) {
+        p.x = a.x0 + (int32_t)gen_range(d, (uint32_t)(a.x1 - a.x0 + 1));
+        p.y = a.y0 + (int32_t)gen_range(d, (uint32_t)(a.y1 - a.y0 + 1));
+        return p;
+    }
+    const uint32_t* w = move_words(d);
     p.x = a.x0 + (int32_t)uniform_u32(w[0], (uint32_t)(a.x1 - a.x0 + 1));
     p.y = a.y0 + (int32_t)uniform_u32(w[1], (uint32_t)(a.y1 - a.y0 + 1));
     return p;
@@ -323,7 +363,14 @@ static point move_agent_from(const epicpu_sim* s, const citizen* c, point cell, 
         if (rect_contains(c->current_area, p.x, p.y) && in_grid(s, p.x, p.y) && is_vacant(s, p)) cand[n++] = p;
     }
     point nc = cell;
-    if (n > 0) nc = cand[uniform_u32(move_words(d)[2], n)];
+    if (d->ref) {
+        /* IteratorRandom::choose on a Filter (size hint lower bound 0): reservoir sampling,
+           element k replaces the choice when gen_range(0..k) == 0 */
+        for (uint32_t k = 0; k < n; k++)
+            if (gen_range(d, k + 1) == 0) nc = cand[k];
+    } else if (n > 0) {
+        nc = cand[uniform_u32(move_words(d)[2], n)];
+    }
     return move_agent(s, cell, nc);
 }
 
@@ -357,6 +404,11 @@ static int on_susceptible(const epicpu_sim* s, const citizen* c, point cell, dra
         if (nb->state != ST_I || nb->hospitalized) continue;
         /* rate(occupant) = get_current_transmission_rate(day + immunity), citizen/mod.rs:182-185 */
         int win = transmission_window(&s->cfg, immunity_day(nb));
+        if (d->ref) { /* short-circuit find: one gen_bool per infectious neighbour, rate 0 included */
+            uint64_t thr = win == 0 ? 0 : (win == 1 ? s->thr_regular : s->thr_high);
+            if (bernoulli(thr, next_u64(d))) return 1;
+            continue;
+        }
         if (win == 0) continue; /* gen_bool(0.0) never fires */
         uint64_t thr = win == 1 ? s->thr_regular : s->thr_high;
         if (bernoulli(thr, infect_u64(d, k))) return 1;
@@ -370,10 +422,11 @@ static void update_infection_dynamics(const epicpu_sim* s, citizen* c, point cel
     if (c->state == ST_S) {
         if (on_susceptible(s, c, cell, d)) { c->state = ST_E; c->at_hour = h; }
     } else if (c->state == ST_E) {
-        const uint32_t* w = disease_words(d);
-        int32_t rf = (int32_t)uniform_u32(w[0], 3) - 1; /* RANGE_FOR_EXPOSED, constants.rs:43 */
+        const uint32_t* w = d->ref ? NULL : disease_words(d);
+        /* RANGE_FOR_EXPOSED, constants.rs:43 */
+        int32_t rf = (int32_t)(d->ref ? gen_range(d, 3) : uniform_u32(w[0], 3)) - 1;
         if (h - c->at_hour >= (uint32_t)((int32_t)cfg->exposed_duration + rf)) {
-            int symptoms = bernoulli(s->thr_symptoms, u64_of(w[2], w[3]));
+            int symptoms = bernoulli(s->thr_symptoms, d->ref ? next_u64(d) : u64_of(w[2], w[3]));
             c->state = ST_I;
             c->infection_day = 0;
             if (symptoms) { c->severity = SEV_PRE; c->at_hour = h; }
@@ -381,8 +434,8 @@ static void update_infection_dynamics(const epicpu_sim* s, citizen* c, point cel
         }
     } else if (c->state == ST_I) {
         if (c->severity == SEV_PRE && h - c->at_hour >= cfg->pre_symptomatic_duration) {
-            const uint32_t* w = disease_words(d);
-            c->severity = bernoulli(s->thr_severe, u64_of(w[2], w[3])) ? SEV_SEVERE : SEV_MILD;
+            const uint64_t v = d->ref ? next_u64(d) : u64_of(disease_words(d)[2], disease_words(d)[3]);
+            c->severity = bernoulli(s->thr_severe, v) ? SEV_SEVERE : SEV_MILD;
             c->at_hour = 0;
         }
     }
@@ -415,8 +468,8 @@ static point deceased(const epicpu_sim* s, citizen* c, point cell, draws* d) {
         if (c->severity == SEV_ASYM && c->infection_day == 9) c->state = ST_R;       /* ASYMPTOMATIC_LAST_DAY */
         else if (c->severity == SEV_MILD && c->infection_day == 12) c->state = ST_R; /* MILD_INFECTED_LAST_DAY */
         else if (c->severity == SEV_SEVERE && c->infection_day == s->cfg.last_day) {
-            const uint32_t* w = disease_words(d);
-            c->state = bernoulli(s->thr_death, u64_of(w[2], w[3])) ? ST_D : ST_R;
+            const uint64_t v = d->ref ? next_u64(d) : u64_of(disease_words(d)[2], disease_words(d)[3]);
+            c->state = bernoulli(s->thr_death, v) ? ST_D : ST_R;
         }
         if (c->state != ST_I) { c->severity = 0; c->infection_day = 0; c->at_hour = 0; }
     }
@@ -485,6 +538,7 @@ static point routine(const epicpu_sim* s, citizen* c, point cell, uint32_t h) {
     draws d;
     memset(&d, 0, sizeof d);
     d.s = s; d.id = c->id; d.hour = h; d.infect_block = -1;
+    d.ref = s->ref_order; d.site = SITE_REF_STREAM; d.spos = 4;
     point nc = cell;
     uint32_t hd = h % 24;
     if (hd == 0) {
@@ -611,7 +665,19 @@ static int simulate(epicpu_sim* s, uint32_t h) {
     /* Sequential merge. Iterating in id order makes "first inserted wins" equal to
        "lowest agent id wins". Targets are vacant in the snapshot or the agent's own
        cell, so a loser's old cell is always free in upcoming. */
-    for (uint32_t i = 0; i < s->n; i++) {
+    if (s->ref_order) { /* merge in a uniformly random order (stands in for hash order) */
+        draws d;
+        memset(&d, 0, sizeof d);
+        d.s = s; d.id = 0; d.hour = h; d.site = SITE_REF_MERGE; d.spos = 4;
+        for (uint32_t i = 0; i < s->n; i++) s->perm[i] = i;
+        for (uint32_t i = s->n; i > 1; i--) {
+            uint32_t j = gen_range(&d, i), t = s->perm[i - 1];
+            s->perm[i - 1] = s->perm[j];
+            s->perm[j] = t;
+        }
+    }
+    for (uint32_t k = 0; k < s->n; k++) {
+        const uint32_t i = s->ref_order ? s->perm[k] : k;
         point oc = s->pos[i], nc = s->tgt[i];
         uint32_t ci = cell_index(s, nc.x, nc.y);
         if (s->upcoming[ci] == 0) {
@@ -875,7 +941,7 @@ epicpu_sim* epicpu_create(const epicpu_config* cfg, char* err, size_t errlen) {
 void epicpu_destroy(epicpu_sim* s) {
     if (!s) return;
     free(s->ag); free(s->upd); free(s->pos); free(s->tgt);
-    free(s->current); free(s->upcoming); free(s->events);
+    free(s->current); free(s->upcoming); free(s->events); free(s->perm);
     free(s);
 }
 

@@ -94,6 +94,12 @@ typedef struct epicpu_sim epicpu_sim;
 /* Returns NULL on error and writes the message (mirroring the reference panic text). */
 epicpu_sim* epicpu_create(const epicpu_config* cfg, char* err, size_t errlen);
 void epicpu_destroy(epicpu_sim* s);
+/* Reference-order mode (on = 1), for distributional checks only: each agent-hour consumes
+   one sequential random stream in the reference's call order (rand 0.8.5 gen_range with
+   rejection, IteratorRandom::choose reservoir sampling, short-circuit infection find with a
+   gen_bool per infectious neighbour) and the merge visits agents in a random order per
+   hour (standing in for hash order) instead of id order. Not bit-comparable with the GPU. */
+int epicpu_set_reference_order(epicpu_sim* s, int on);
 /* Threads for the per-agent map (the merge stays sequential); results do not depend on it. */
 void epicpu_set_threads(epicpu_sim* s, uint32_t threads);
 /* Runs up to n_hours hours; rows gets 7 u32 per hour (hour,S,E,I,H,R,D). */

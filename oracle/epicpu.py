@@ -93,6 +93,7 @@ def lib():
         L.epicpu_create.argtypes = [_P(EpicpuConfig), ctypes.c_char_p, ctypes.c_size_t]
         L.epicpu_destroy.argtypes = [ctypes.c_void_p]
         L.epicpu_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        L.epicpu_set_reference_order.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.epicpu_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_uint32), _P(ctypes.c_uint32),
                                   _P(ctypes.c_int)]
         L.epicpu_read_agents.argtypes = [ctypes.c_void_p, _P(EpicpuAgents)]
@@ -222,6 +223,12 @@ class OracleSim:
 
     def set_threads(self, threads):
         lib().epicpu_set_threads(self.h, threads)
+
+    def set_reference_order(self, on=True):
+        """Sequential per-agent-hour streams in the reference's call order and a random
+        merge order (distributional checks; not bit-comparable with deterministic mode)."""
+        if lib().epicpu_set_reference_order(self.h, 1 if on else 0) != 0:
+            raise MemoryError("epicpu_set_reference_order")
 
     def close(self):
         if self.h:
