@@ -1,10 +1,11 @@
 """Summaries of rocprofv3 output for the hour-step kernels.
 
-    python -m epirust_amd.prof_summary trace <kernel_stats.csv> [--command "..."]
-    python -m epirust_amd.prof_summary traffic <fetch_dir> <write_dir> --agents N --grid G [--command "..."]
+    python -m epirust_amd.prof_summary trace <kernel_stats.csv> [<kernel_trace.csv> --hours H] [--command "..."]
+    python -m epirust_amd.prof_summary traffic <fetch.csv> <write.csv> --agents N --grid G --hours H [--command "..."]
 
 `trace` turns a `--kernel-trace --stats` kernel_stats.csv into a markdown table (calls,
-average / min / max duration per kernel). `traffic` reads two `--pmc FETCH_SIZE` and
+average / min / max duration per kernel) and, given the kernel_trace.csv, a table of the
+hour kernels' mean durations per hour of day. `traffic` reads two `--pmc FETCH_SIZE` and
 `--pmc WRITE_SIZE` counter_collection.csv files of `bench.py --profile-hours H` (one
 dispatch per kernel and hour, then a 64 MiB device-to-device copy). It scales each
 kernel's KiB by the ratio the copy dispatch needs to read/write exactly 64 MiB, reports
@@ -14,21 +15,63 @@ import argparse
 import collections
 import csv
 import json
-import os
+import re
 import sys
 
-SHORT = {"k_step": "k_step", "k_commit": "k_commit", "k_epilogue": "k_epilogue",
-         "k_hospital_scan": "k_hospital_scan", "copyBuffer": "copy (calibration)", "k_fill_u64": "k_fill_u64"}
+HOUR_KERNELS = ("k_hospital_scan", "k_step", "k_epilogue", "k_commit")
 
 
 def short(name):
-    for k, v in SHORT.items():
-        if k in name:
-            return v
+    m = re.search(r"\b(k_\w+)\(", name)
+    if m:
+        return m.group(1)
+    if "copyBuffer" in name:
+        return "copy (calibration)"
     return name[:40]
 
 
-def trace(path, command):
+def by_hour_of_day(trace_csv, hours):
+    """Mean duration per kernel for each hour of day over the first `hours` hours of the
+    trace (hours are counted by k_step dispatches; the hour-0 hospital scan runs just
+    before the step of hour 24k), with the wall time from one step to the next."""
+    ks = []
+    for r in csv.DictReader(open(trace_csv)):
+        k = short(r["Kernel_Name"])
+        if k in HOUR_KERNELS:
+            ks.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), k))
+    ks.sort()
+    dur = collections.defaultdict(lambda: collections.defaultdict(list))
+    start = {}
+    hour = 0
+    for s, e, k in ks:
+        if k == "k_hospital_scan":
+            dur[(hour + 1) % 24][k].append((e - s) / 1e3)
+            continue
+        if k == "k_step":
+            hour += 1
+            if hour > hours:
+                break
+            start[hour] = s
+        dur[hour % 24][k].append((e - s) / 1e3)
+    wall = collections.defaultdict(list)
+    for h in range(1, hours):
+        if h + 1 in start:
+            wall[h % 24].append((start[h + 1] - start[h]) / 1e3)
+    out = ["| hour of day | " + " | ".join(HOUR_KERNELS) + " | kernels µs | step-to-step µs |",
+           "|---" * (len(HOUR_KERNELS) + 3) + "|"]
+    tot = 0.0
+    for hd in range(24):
+        means = [sum(dur[hd][k]) / len(dur[hd][k]) if dur[hd][k] else 0.0 for k in HOUR_KERNELS]
+        w = sorted(wall[hd])[len(wall[hd]) // 2] if wall[hd] else 0.0
+        tot += sum(means)
+        out.append("| %d | " % hd + " | ".join("%.1f" % m for m in means) + " | %.1f | %.1f |" % (sum(means), w))
+    out.append("")
+    out.append("Mean kernel time per simulated hour: %.1f µs. Step-to-step times include the profiler's "
+               "own per-dispatch overhead; the untraced hour is shorter." % (tot / 24))
+    return "\n".join(out) + "\n"
+
+
+def trace(path, command, trace_csv=None, hours=0):
     rows = list(csv.DictReader(open(path)))
     out = ["# Kernel trace summary", "", "Command: `%s`" % command, "",
            "| kernel | calls | average µs | min µs | max µs | total ms | share |", "|---|---|---|---|---|---|---|"]
@@ -37,7 +80,10 @@ def trace(path, command):
         out.append("| %s | %s | %.2f | %.2f | %.2f | %.3f | %.1f%% |" % (
             short(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
             float(r["MaxNs"]) / 1e3, float(r["TotalDurationNs"]) / 1e6, 100.0 * float(r["TotalDurationNs"]) / total))
-    return "\n".join(out) + "\n"
+    text = "\n".join(out) + "\n"
+    if trace_csv:
+        text += "\n## By hour of day (first %d simulated hours)\n\n" % hours + by_hour_of_day(trace_csv, hours)
+    return text
 
 
 def _counter(path, counter):
@@ -98,7 +144,7 @@ def main(argv):
     ap.add_argument("--json", default="")
     a = ap.parse_args(argv)
     if a.mode == "trace":
-        text = trace(a.paths[0], a.command)
+        text = trace(a.paths[0], a.command, a.paths[1] if len(a.paths) > 1 else None, a.hours)
     else:
         text, js = traffic(a.paths[0], a.paths[1], a.agents, a.grid, a.hours, a.command)
         if a.json:
