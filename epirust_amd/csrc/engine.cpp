@@ -224,7 +224,7 @@ int enqueue_hour(epi_sim* s, uint32_t next_hour) {
   if (next_hour % 24u == 0) HIP_TRY(launch_hospital_scan(s->v, s->shape, s->stream));
   HIP_TRY(launch_step(s->v, s->shape, s->stream));
   HIP_TRY(launch_epilogue(s->v, s->shape, s->stream));
-  HIP_TRY(launch_commit(s->v, s->shape, s->stream));
+  if (hour_has_commit(next_hour)) HIP_TRY(launch_commit(s->v, s->shape, next_hour, s->stream));
   return EPI_OK;
 }
 
@@ -367,6 +367,7 @@ int step_impl(epi_sim* s, uint32_t n_hours, epi_counts* rows, uint32_t row_strid
   const uint32_t target = static_cast<uint32_t>(std::min<uint64_t>(want, s->max_hour));
   int rc = advance(s, target);
   if (rc != EPI_OK) return rc;
+  HIP_TRY(launch_apply_sweep(s->v, s->shape, s->stream));  // agent state includes the last hour's sweep
   rc = fetch_states(s);
   if (rc != EPI_OK) return rc;
   s->all_stopped = true;
@@ -469,6 +470,10 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     D.office_sy = G.office_sy;
     D.office_nx = std::max(G.office_nx, 1);
     D.n_offices = std::max(G.n_offices, 1u);
+    D.fd_houses = make_fastdiv(D.n_houses);
+    D.fd_house_nx = make_fastdiv(static_cast<uint32_t>(D.house_nx));
+    D.fd_offices = make_fastdiv(D.n_offices);
+    D.fd_office_nx = make_fastdiv(static_cast<uint32_t>(D.office_nx));
     D.regular_start = c.regular_transmission_start_day;
     D.high_start = c.high_transmission_start_day;
     D.last_day = c.last_day;
@@ -564,6 +569,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     SimState st;
     std::memset(&st, 0, sizeof st);
     st.active = (s->cfg[i].hours > 1 && D.n > 0) ? 1u : 0u;
+    st.commit_hour[1] = st.active;  // hour 1 (odd); it moves nobody, but the slot must be right
     st.first_vacant = ~0u;
     s->state_h[i] = st;
     if (!host_init) {
@@ -975,7 +981,8 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
       if (launch_step(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       if (launch_epilogue(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       (void)hipEventRecord(e[2], sim->stream);
-      if (launch_commit(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      if (hour_has_commit(h) && launch_commit(sim->v, sim->shape, h, sim->stream) != hipSuccess)
+        rc = fail(EPI_ERR_DEVICE, "launch");
       (void)hipEventRecord(e[3], sim->stream);
       ++sim->host_hour;
     }
@@ -1000,6 +1007,7 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
     launches[k] = cnt[k];
   }
   if (hours_run) *hours_run = ran;
+  HIP_TRY(launch_apply_sweep(sim->v, sim->shape, sim->stream));
   // keep row accounting consistent for later epi_step calls
   rc = fetch_states(sim);
   if (rc != EPI_OK) return rc;

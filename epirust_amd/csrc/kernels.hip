@@ -3,14 +3,20 @@
 // One hour of EpiRust's standalone engine (engine/src/allocation_map.rs:67-134 and
 // :306-337, engine/src/citizen/mod.rs:227-454, default_disease_handler.rs:31-103) is
 //   [hour-of-day 0 only] k_hospital_scan : first vacant hospital cell of the snapshot
-//   k_step   : per agent, routine + disease update against the start-of-hour grid;
-//              state written in place (it is read by no other agent), the chosen cell
-//              claimed with a 64-bit atomicMin keyed (hour, agent id); S/E/I/H/R/D
-//              counted with wavefront ballots into per-block partials.
+//   k_step   : per agent, the previous hour's vaccination / lockdown sweep, then routine +
+//              disease update against the start-of-hour grid; state written in place (it
+//              is read by no other agent), the chosen cell claimed with a 64-bit atomicMin
+//              keyed (hour, agent id); S/E/I/H/R/D counted with wavefront ballots into
+//              per-block partials.
 //   k_epilogue: one block per simulation reduces the partials, writes the hour's row and
 //              runs the intervention triggers and the stop rule on device.
-//   k_commit : per agent, the claim winner moves, grid bytes are updated, and the
-//              vaccination / lockdown sweeps decided by k_step's epilogue are applied.
+//   k_commit : per agent, the claim winner moves and grid bytes are updated. It reads
+//              nothing the same hour's epilogue writes (its hour comes from a slot the
+//              previous epilogue filled), so the two need not be ordered. Hours of day
+//              1..6 move nobody and launch no commit.
+// A sweep decided at hour h (allocation_map.rs:306-337) is applied by k_step of hour h+1
+// before the routine, which is what the reference's next hour sees; sweeps are
+// idempotent, so k_apply_sweep re-applies the last one before the host reads agents.
 // Every kernel is batched: blocks map to (simulation, agent range) through a table, so
 // many independent replicates share one launch. Nothing here uses MFMA: the step is
 // integer work bound by HBM traffic.
@@ -31,11 +37,6 @@ __device__ __forceinline__ bool contains(const Rect& a, int32_t x, int32_t y) {
 }
 __device__ __forceinline__ bool same_rect(const Rect& a, const Rect& b) {
   return a.x0 == b.x0 && a.y0 == b.y0 && a.x1 == b.x1 && a.y1 == b.y1;
-}
-__device__ __forceinline__ Rect building(int32_t sx, int32_t sy, int32_t nx, int32_t size, uint32_t k) {
-  const int32_t x0 = sx + static_cast<int32_t>(k % static_cast<uint32_t>(nx)) * size;
-  const int32_t y0 = sy + static_cast<int32_t>(k / static_cast<uint32_t>(nx)) * size;
-  return Rect{x0, y0, x0 + size - 1, y0 + size - 1};
 }
 
 // Moore neighbour order (point.rs:57-61): (-1,-1) (0,-1) (1,-1) (-1,0) (1,0) (-1,1) (0,1) (1,1)
@@ -93,6 +94,8 @@ struct Agent {
   uint32_t id, h, w, tm, wsa;
   int32_t x, y;                     // current cell
   int32_t tx, ty;                   // chosen cell
+  int32_t hx = 0, hy = 0;           // home house origin (house id % H)
+  int32_t ox = 0, oy = 0;           // office origin (office id % O)
   // The hour's movement and disease draw blocks, computed once up front (one inlined
   // Philox each) for the hours / states that can use them.
   u32x4 mv{{0, 0, 0, 0}}, dz{{0, 0, 0, 0}};
@@ -119,11 +122,19 @@ struct Agent {
   __device__ __forceinline__ bool can_move() const {  // citizen/mod.rs:452-454
     return !(symptomatic() || (w & F_HOSP) || state() == ST_D || (w & F_ISO));
   }
-  __device__ __forceinline__ Rect home() const {
-    return building(D.house_sx, D.house_sy, D.house_nx, 2, id % D.n_houses);
+  // Home house id % H and office id % O, tiled row-major (area.rs:95-117), once per hour
+  // with the division constants of the tiling.
+  __device__ __forceinline__ void locate_buildings() {
+    const uint32_t hk = fast_mod(id, D.fd_houses), hrow = fast_div(hk, D.fd_house_nx);
+    hx = D.house_sx + static_cast<int32_t>(hk - hrow * D.fd_house_nx.d) * 2;
+    hy = D.house_sy + static_cast<int32_t>(hrow) * 2;
+    const uint32_t ok = fast_mod(id, D.fd_offices), orow = fast_div(ok, D.fd_office_nx);
+    ox = D.office_sx + static_cast<int32_t>(ok - orow * D.fd_office_nx.d) * 10;
+    oy = D.office_sy + static_cast<int32_t>(orow) * 10;
   }
+  __device__ __forceinline__ Rect home() const { return Rect{hx, hy, hx + 1, hy + 1}; }
   __device__ __forceinline__ Rect work() const {  // non-workers' work location is home
-    return ws() == WS_NA ? home() : building(D.office_sx, D.office_sy, D.office_nx, 10, id % D.n_offices);
+    return ws() == WS_NA ? home() : Rect{ox, oy, ox + 9, oy + 9};
   }
   __device__ __forceinline__ Rect area_rect(uint32_t tag) const {
     switch (tag) {
@@ -195,13 +206,36 @@ struct Agent {
     if (vacant(px, py)) { tx = px; ty = py; }
   }
 
-  // citizen/mod.rs:415-432
-  __device__ __forceinline__ void move_agent_from() {
-    if (!can_move()) return;
-    const Rect a = current_area();
+  // The hour's movement primitive. Every path of perform_movements ends in at most one of
+  //   ACT_LOCAL: move_agent_from (citizen/mod.rs:415-432): from the current cell, or from a
+  //              random point of the current area when outside it, to a random vacant
+  //              Moore neighbour inside the area;
+  //   ACT_TELE : goto_area for a working citizen (citizen/mod.rs:367-395): a random point
+  //              of the target area if vacant (a non-worker's goto_area is ACT_LOCAL).
+  // Lanes choose their primitive first and one shared copy runs, so a wavefront mixing work
+  // statuses pays for one window load and one neighbour scan.
+  enum : uint32_t { ACT_NONE = 0, ACT_LOCAL = 1, ACT_TELE = 2 };
+
+  // goto_area's gate for a working citizen: can_move, or the symptomatic-at-work override.
+  __device__ __forceinline__ uint32_t tele_act(const Rect& target) const {
+    const uint32_t s = ws();
+    const bool override_movement = (s == WS_NORMAL || s == WS_ESSENTIAL) && contains(work(), x, y) &&
+                                   same_rect(target, home()) && symptomatic();
+    return (can_move() || override_movement) ? ACT_TELE : ACT_NONE;
+  }
+
+  __device__ __forceinline__ void move(uint32_t act, const Rect& target) {
+    if (act == ACT_NONE) return;
+    const bool tele = act == ACT_TELE;
+    if (!tele && !can_move()) return;
+    const Rect a = tele ? target : current_area();
     int32_t lx = x, ly = y;
-    if (!contains(a, x, y)) random_point(a, lx, ly);  // teleport origin, vacancy not checked
+    if (tele || !contains(a, x, y)) random_point(a, lx, ly);  // teleport origin, vacancy not checked
     load_window(lx, ly);
+    if (tele) {  // vacancy of the point; the window also serves the infection scan around it
+      if (wbyte<0>(0) == 0) { tx = lx; ty = ly; }
+      return;
+    }
     uint32_t cand = 0, n = 0;
     for_moore([&](auto kc) {
       constexpr int k = decltype(kc)::value;
@@ -217,25 +251,6 @@ struct Agent {
     const int k = __builtin_ctz(cand);
     tx = lx + moore_dx(k);
     ty = ly + moore_dy(k);
-  }
-
-  // citizen/mod.rs:367-395
-  __device__ __forceinline__ void goto_area(const Rect& target) {
-    bool override_movement = false;
-    const uint32_t s = ws();
-    if (s == WS_NORMAL || s == WS_ESSENTIAL) {
-      const Rect wk = work();
-      if (contains(wk, x, y) && same_rect(target, home()) && symptomatic()) override_movement = true;
-    }
-    if (!can_move() && !override_movement) return;
-    if (s != WS_NA) {
-      int32_t px, py;
-      random_point(target, px, py);
-      load_window(px, py);  // vacancy of the point + the infection scan around it
-      if (wbyte<0>(0) == 0) { tx = px; ty = py; }
-      return;
-    }
-    move_agent_from();
   }
 
   // default_disease_handler.rs:64-86 (OR over fixed per-neighbour draws)
@@ -309,56 +324,66 @@ struct Agent {
     }
   }
 
-  // citizen/mod.rs:257-349
+  // citizen/mod.rs:257-349: choose the primitive and the area it leads to, run it, then the
+  // disease update (skipped by the hospital-staff quarantine transitions, which return early).
   __device__ __forceinline__ void perform_movements(uint32_t hd, const uint32_t* __restrict__ tm_arr, uint64_t gi) {
     const uint32_t s = ws();
+    uint32_t act = ACT_NONE, new_area = ~0u;
+    Rect target = D.housing;
+    bool disease = true;
     if (s == WS_NORMAL || s == WS_ESSENTIAL) {
       if (hd == 7 || hd == 17) {
-        if (w & F_PT) { goto_area(D.transport); set_area(AREA_TRANSPORT); }
-        else move_agent_from();
+        if (w & F_PT) {
+          target = D.transport;
+          act = tele_act(target);
+          new_area = AREA_TRANSPORT;
+        } else {
+          act = ACT_LOCAL;
+        }
       } else if (hd == 8) {
-        goto_area(work());
-        set_area(AREA_WORK);
+        target = work();
+        act = tele_act(target);
+        new_area = AREA_WORK;
       } else if (hd == 16) {
-        goto_area(home());
-        set_area(AREA_HOME);
+        target = home();
+        act = tele_act(target);
+        new_area = AREA_HOME;
       } else {
-        move_agent_from();
+        act = ACT_LOCAL;
       }
     } else if (s == WS_STAFF) {
       const uint32_t since = h > wsa ? h - wsa : 0u;  // saturating_sub
-      if (since == 24u * 14u) { w |= F_WQ; return; }
-      if (since == 24u * 14u * 2u) {
-        goto_area(home());
-        set_area(AREA_HOME);
+      if (since == 24u * 14u) {
+        w |= F_WQ;
+        disease = false;
+      } else if (since == 24u * 14u * 2u) {
+        target = home();
+        act = tele_act(target);
+        new_area = AREA_HOME;
         wsa = h + 24u * 14u;
-        return;
-      }
-      if (hd == 8) {
+        disease = false;
+      } else if (hd == 8) {
         const uint32_t ver = S.hosp_version;
         if (!same_rect(current_area(), D.hosp[ver]) && wsa <= h) {
-          goto_area(D.hosp[ver]);
-          set_area(AREA_HOSP0 + ver);
+          target = D.hosp[ver];
+          act = tele_act(target);
+          new_area = AREA_HOSP0 + ver;
           wsa = h;
         }
         w &= ~F_WQ;
       } else if (hd == 16) {
         w |= F_WQ;
-      } else if (!(w & F_WQ) && can_move()) {
-        move_agent_from();
+      } else if (!(w & F_WQ)) {
+        act = ACT_LOCAL;
       }
-    } else {  // NA
-      if (hd == 8) {
-        goto_area(D.housing);
-        set_area(AREA_HOUSING);
-      } else if (hd == 12) {
-        goto_area(home());
-        set_area(AREA_HOME);
-      } else {
-        move_agent_from();
-      }
+    } else {  // NA: goto_area of a non-worker is move_agent_from in the current area
+      act = ACT_LOCAL;
+      if (hd == 8) new_area = AREA_HOUSING;
+      else if (hd == 12) new_area = AREA_HOME;
     }
-    update_infection_dynamics(tm_arr, gi);
+    move(act, target);
+    if (new_area != ~0u) set_area(new_area);
+    if (disease) update_infection_dynamics(tm_arr, gi);
   }
 
   // citizen/mod.rs:351-365 with allocation_map.rs:144-150
@@ -421,18 +446,18 @@ __device__ void hour_epilogue(const DevView& v, const SimDesc& D, SimState* S, u
   const uint32_t infected = c[2];
   uint32_t sweep = 0;
   if (D.n_vacc) {
-    // schedule sorted by hour, one entry per hour (the last config entry wins)
-    uint32_t lo = 0, hi = D.n_vacc;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (v.vacc_hour[D.vacc_off + mid] < h) lo = mid + 1; else hi = mid;
-    }
-    if (lo < D.n_vacc && v.vacc_hour[D.vacc_off + lo] == h) {
+    // schedule sorted by hour, one entry per hour (the last config entry wins); hours
+    // advance by one per call, so a cursor replaces a search
+    uint32_t k = S->vacc_next;
+    while (k < D.n_vacc && v.vacc_hour[D.vacc_off + k] < h) ++k;
+    if (k < D.n_vacc && v.vacc_hour[D.vacc_off + k] == h) {
       sweep |= SWEEP_VACC;
-      S->vacc_thr = v.vacc_thr[D.vacc_off + lo];
+      S->vacc_thr = v.vacc_thr[D.vacc_off + k];
       S->vacc_hour = h;
       add_event(v, D, S, h, EV_VACCINATION);
+      ++k;
     }
+    S->vacc_next = k;
   }
   if (D.has_lockdown && !S->locked && h % 24u == 0 && infected > D.lockdown_at) {
     S->locked = 1;
@@ -455,8 +480,53 @@ __device__ void hour_epilogue(const DevView& v, const SimDesc& D, SimState* S, u
   if ((c[1] == 0 && c[2] == 0 && c[3] == 0) || h + 1 >= D.hours) S->active = 0;
   S->hour = h;
   S->rows_written = h;
-  S->commit_pending = 1;
   S->first_vacant = ~0u;  // this hour's scan is consumed; the next one starts fresh
+  S->commit_hour[(h + 1u) & 1u] = S->active ? h + 1u : 0u;
+}
+
+// The sweep decided at the previous hour (allocation_map.rs:349-387): vaccination of
+// susceptible agents (a Bernoulli keyed by the vaccination hour), lockdown isolation of
+// everyone but essential workers, unlock. Applying it twice gives the same word.
+__device__ __forceinline__ uint32_t apply_sweep(uint32_t w, uint32_t sweep, uint32_t local, const SimState& S,
+                                                const SimDesc& D) {
+  if ((sweep & SWEEP_VACC) && fld(w, B_STATE, M_STATE) == ST_S) {
+    const u32x4 d = philox4x32_10(local, S.vacc_hour, SITE_VACCINATE, 0, D.seed_lo, D.seed_hi);
+    if (bernoulli(S.vacc_thr, join64(d.w[0], d.w[1]))) w |= F_VACC;
+  }
+  if ((sweep & SWEEP_LOCK) && fld(w, B_WS, M_WS) != WS_ESSENTIAL) w |= F_ISO;
+  if (sweep & SWEEP_UNLOCK) w &= ~F_ISO;
+  return w;
+}
+
+// Block-wide hour epilogue of one simulation: reduce the hour's per-block partial counts
+// (hours of day 2..6 repeat the previous row: nothing changed), then hour_epilogue on
+// thread 0. `h` must be read before any thread can reach hour_epilogue.
+__device__ void run_epilogue(const DevView& v, const SimDesc& D, SimState* S, uint32_t h) {
+  __shared__ uint32_t s_sum[6];
+  if (h % 24u >= 2 && h % 24u <= 6) {
+    if (threadIdx.x == 0) {
+      const Counts7 prev = v.rows[D.row_off + (h - 2)];
+      const uint32_t c[6] = {prev.s, prev.e, prev.i, prev.h, prev.r, prev.d};
+      hour_epilogue(v, D, S, h, c);
+    }
+    return;
+  }
+  if (threadIdx.x < 6) s_sum[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t acc[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t b = threadIdx.x; b < D.n_blocks; b += blockDim.x) {
+    const uint32_t* bc = v.block_counts + 6u * (D.block_begin + b);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[k] += bc[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    uint32_t x = acc[k];
+    for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && x) atomicAdd(&s_sum[k], x);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) hour_epilogue(v, D, S, h, s_sum);
 }
 
 __global__ void k_step(DevView v) {
@@ -467,12 +537,21 @@ __global__ void k_step(DevView v) {
   if (!S->active) return;  // uniform over the block
   const uint32_t h = S->hour + 1;
   const uint32_t hd = h % 24u;
-  // Hours of day 2..6 change nothing: hour 1 already set every non-staff current area to
-  // home, and no movement or disease update runs while asleep (citizen/mod.rs:244-248).
-  // k_epilogue repeats the previous hour's counts for them.
-  if (hd >= 2 && hd <= 6) return;
+  const uint32_t sweep = S->sweep;  // decided at hour h - 1
   const uint32_t local = first_lane(bm.first) + threadIdx.x;
   const bool valid = local < D.n;
+  // Hours of day 2..6 change nothing: hour 1 already set every non-staff current area to
+  // home, and no movement or disease update runs while asleep (citizen/mod.rs:244-248).
+  // k_epilogue repeats the previous hour's counts for them; only a pending sweep is applied.
+  if (hd >= 2 && hd <= 6) {
+    if (sweep && valid) {
+      const uint64_t gi = D.agent_off + local;
+      const uint32_t w = v.st[gi];
+      const uint32_t nw = apply_sweep(w, sweep, local, *S, D);
+      if (nw != w) v.st[gi] = nw;
+    }
+    return;
+  }
   const uint8_t* occ = v.occ + D.cell_off;
 
   uint32_t cat = 6;
@@ -481,14 +560,16 @@ __global__ void k_step(DevView v) {
     Agent a(D, *S, occ);
     a.id = local;
     a.h = h;
-    a.w = v.st[gi];
+    a.locate_buildings();
+    const uint32_t w_stored = v.st[gi];
+    a.w = sweep ? apply_sweep(w_stored, sweep, local, *S, D) : w_stored;
     const uint32_t p = v.pos[gi];
     a.x = static_cast<int32_t>(p & 0xFFFFu);
     a.y = static_cast<int32_t>(p >> 16);
     a.tx = a.x;
     a.ty = a.y;
     a.tm = 0;
-    const uint32_t w0 = a.w;
+    const uint32_t w0 = w_stored;
     const bool staff = a.ws() == WS_STAFF;
     a.wsa = staff ? v.wsa[gi] : 0u;
     const uint32_t wsa0 = a.wsa;
@@ -546,55 +627,25 @@ __global__ void k_step(DevView v) {
   if (threadIdx.x < 6) v.block_counts[6u * blockIdx.x + threadIdx.x] = s_counts[threadIdx.x];
 }
 
-// One block per simulation: reduce the hour's block partial counts, then the epilogue.
-// Runs every hour; for a simulation that did not step this hour it only retires the
-// previous hour's commit.
+// One block per simulation: the hour's epilogue.
 __global__ void k_epilogue(DevView v) {
   const uint32_t sim = blockIdx.x;
-  const SimDesc& D = v.desc[sim];
   SimState* S = v.state + sim;
-  if (!S->active) {
-    if (threadIdx.x == 0) S->commit_pending = 0;
+  if (!S->active) {  // uniform; the slot of the commit two hours on must say "none"
+    if (threadIdx.x == 0) S->commit_hour[S->hour & 1u] = 0u;
     return;
   }
-  __shared__ uint32_t s_sum[6];
-  const uint32_t h = S->hour + 1;
-  if (h % 24u >= 2 && h % 24u <= 6) {  // k_step skipped this hour: its counts equal the last row's
-    if (threadIdx.x == 0) {
-      const Counts7 prev = v.rows[D.row_off + (h - 2)];
-      const uint32_t c[6] = {prev.s, prev.e, prev.i, prev.h, prev.r, prev.d};
-      hour_epilogue(v, D, S, h, c);
-    }
-    return;
-  }
-  if (threadIdx.x < 6) s_sum[threadIdx.x] = 0;
-  __syncthreads();
-  uint32_t acc[6] = {0, 0, 0, 0, 0, 0};
-  for (uint32_t b = threadIdx.x; b < D.n_blocks; b += blockDim.x) {
-    const uint32_t* bc = v.block_counts + 6u * (D.block_begin + b);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) acc[k] += bc[k];
-  }
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    uint32_t x = acc[k];
-    for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0 && x) atomicAdd(&s_sum[k], x);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) hour_epilogue(v, D, S, h, s_sum);
+  run_epilogue(v, v.desc[sim], S, S->hour + 1);
 }
 
-__global__ void k_commit(DevView v) {
+// `parity` is the hour's parity (an hour of day 24k + j has the parity of j).
+__global__ void k_commit(DevView v, uint32_t parity) {
   const BlockRef bm = v.agent_blocks[blockIdx.x];
   const uint32_t sim = first_lane(bm.sim);
   const SimDesc& D = v.desc[sim];
-  SimState* S = v.state + sim;
-  if (!S->commit_pending) return;
-  const uint32_t h = S->hour;
-  const uint32_t sweep = S->sweep;
-  // No agent moves or changes its grid byte at hours of day 1..6, so only a sweep needs work.
-  if (sweep == 0 && h % 24u >= 1 && h % 24u <= 6) return;
+  const SimState* S = v.state + sim;
+  const uint32_t h = S->commit_hour[parity];
+  if (h == 0) return;  // uniform: the simulation did not step this hour
   const uint32_t local = first_lane(bm.first) + threadIdx.x;
   if (local < D.n) {
     const uint64_t gi = D.agent_off + local;
@@ -603,15 +654,6 @@ __global__ void k_commit(DevView v) {
     const uint32_t w = v.st[gi];
     const uint32_t p = v.pos[gi];
     const uint32_t t = v.tgt[gi];
-    uint32_t nw = w;
-    if (sweep) {
-      if ((sweep & SWEEP_VACC) && fld(w, B_STATE, M_STATE) == ST_S) {  // allocation_map.rs:381-387
-        const u32x4 d = philox4x32_10(local, S->vacc_hour, SITE_VACCINATE, 0, D.seed_lo, D.seed_hi);
-        if (bernoulli(S->vacc_thr, join64(d.w[0], d.w[1]))) nw |= F_VACC;
-      }
-      if ((sweep & SWEEP_LOCK) && fld(w, B_WS, M_WS) != WS_ESSENTIAL) nw |= F_ISO;  // :349-356
-      if (sweep & SWEEP_UNLOCK) nw &= ~F_ISO;                                          // :358-365
-    }
     if (w & (F_MOVE | F_DIRTY)) {
       uint8_t* occ = v.occ + D.cell_off;
       const uint64_t old_ci = cell_index(D, static_cast<int32_t>(p & 0xFFFFu), static_cast<int32_t>(p >> 16));
@@ -627,10 +669,25 @@ __global__ void k_commit(DevView v) {
         }
       }
       if (!moved && (w & F_DIRTY)) occ[old_ci] = byte;
-      nw &= ~(F_MOVE | F_DIRTY);
+      v.st[gi] = w & ~(F_MOVE | F_DIRTY);
     }
-    if (nw != w) v.st[gi] = nw;
   }
+}
+
+// Re-applies each simulation's last decided sweep (idempotent) so that agent state read
+// by the host includes it; the next k_step applies it again to the same words.
+__global__ void k_apply_sweep(DevView v) {
+  const BlockRef bm = v.agent_blocks[blockIdx.x];
+  const uint32_t sim = first_lane(bm.sim);
+  const SimDesc& D = v.desc[sim];
+  const SimState* S = v.state + sim;
+  const uint32_t sweep = S->sweep;
+  const uint32_t local = first_lane(bm.first) + threadIdx.x;
+  if (sweep == 0 || local >= D.n) return;
+  const uint64_t gi = D.agent_off + local;
+  const uint32_t w = v.st[gi];
+  const uint32_t nw = apply_sweep(w, sweep, local, *S, D);
+  if (nw != w) v.st[gi] = nw;
 }
 
 // First vacant cell of the current hospital area in row-major order (area.rs:119-146 +
@@ -689,9 +746,15 @@ hipError_t launch_epilogue(const DevView& v, const LaunchShape& s, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_commit(const DevView& v, const LaunchShape& s, hipStream_t stream) {
+hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, hipStream_t stream) {
   if (s.agent_blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_commit, dim3(s.agent_blocks), dim3(s.block), 0, stream, v);
+  hipLaunchKernelGGL(k_commit, dim3(s.agent_blocks), dim3(s.block), 0, stream, v, hour & 1u);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_t stream) {
+  if (s.agent_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply_sweep, dim3(s.agent_blocks), dim3(s.block), 0, stream, v);
   return hipGetLastError();
 }
 

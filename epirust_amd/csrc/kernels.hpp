@@ -37,10 +37,15 @@ struct LaunchShape {
   uint32_t n_sims = 0;
 };
 
+// One hour: [hour of day 0] scan, step, epilogue, then commit (hours of day 0 and 7..23;
+// `hour` only needs the right parity, so a captured day of hours 1..24 replays for any day).
 hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStream_t stream);
 hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream);
 hipError_t launch_epilogue(const DevView& v, const LaunchShape& s, hipStream_t stream);
-hipError_t launch_commit(const DevView& v, const LaunchShape& s, hipStream_t stream);
+hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, hipStream_t stream);
+// Re-applies the last decided sweep of every simulation (idempotent), before agent read-back.
+hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_t stream);
+inline bool hour_has_commit(uint32_t hour) { return hour % 24u == 0 || hour % 24u >= 7; }
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long value, hipStream_t stream);
 
 }  // namespace epi

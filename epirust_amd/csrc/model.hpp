@@ -17,6 +17,8 @@
 #pragma once
 #include <cstdint>
 
+#include "fastdiv.hpp"
+
 namespace epi {
 
 // ---- state word
@@ -89,6 +91,7 @@ struct SimDesc {
   uint32_t n_houses;
   int32_t office_sx, office_sy, office_nx;
   uint32_t n_offices;
+  FastDiv fd_houses, fd_house_nx, fd_offices, fd_office_nx;  // id -> building index -> (column, row)
   // disease (common/src/disease/mod.rs:26-44)
   uint32_t regular_start, high_start, last_day;
   uint32_t exposed_duration, pre_symptomatic_duration;
@@ -106,13 +109,14 @@ struct SimDesc {
 struct SimState {
   uint32_t hour;            // last hour stepped
   uint32_t active;          // 0 once stopped (E=I=H=0) or out of hours
-  uint32_t commit_pending;  // step ran, commit not yet
-  uint32_t sweep;           // SWEEP_* for the commit of `hour`
+  uint32_t commit_hour[2];  // [hour & 1]: the hour the commit of that parity resolves, 0 = none
+  uint32_t sweep;           // SWEEP_* decided at `hour`, applied by the next step (idempotent)
   uint32_t first_vacant;    // row-major index in the hospital, or ~0u
   uint32_t hosp_version;    // 0 after resize_hospital, 1 after increase_hospital_size
   uint32_t locked, zero_infection_hour;
   uint32_t hosp_new_infections, hosp_applied;
   uint32_t vacc_hour;
+  uint32_t vacc_next;       // first schedule entry not yet passed
   uint32_t n_events;
   uint64_t vacc_thr;
   uint32_t rows_written;

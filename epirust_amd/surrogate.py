@@ -262,6 +262,18 @@ class Surrogate:
         return curves, fb, time.perf_counter() - t0
 
 
+def calibrate_threshold(predicted_err, within, target=0.95):
+    """Largest disagreement threshold t such that, simulating every point whose predicted
+    error exceeds t, at least `target` of the points end within the bound (points not
+    simulated count as within only if their surrogate answer is)."""
+    order = np.sort(np.unique(np.concatenate([predicted_err, [0.0]])))[::-1]
+    for t in order:
+        answered = predicted_err <= t
+        if np.mean(~answered | within) >= target:
+            return float(t)
+    return 0.0
+
+
 def nearest_neighbour(train_x, train_curves, query_x):
     """The no-network method: each query takes the curve of its nearest training point."""
     tx = np.asarray(train_x, dtype=np.float64)
@@ -347,6 +359,25 @@ def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=60,
         "with_fallback_fraction_within": float(okf.mean()),
         "with_fallback_max_rel_err_surrogate_points": float(np.max(np.maximum(efp, eff)[~fb])) if (~fb).any() else None,
     })
+    # Calibrated gate: the disagreement threshold is chosen on the even-indexed evaluation
+    # points (the largest threshold that keeps >= 95% of them within the error bound once
+    # the points above it are simulated) and then measured on the odd-indexed points.
+    calib, test = np.arange(0, n_eval, 2), np.arange(1, n_eval, 2)
+    t_star = calibrate_threshold(err_hat[calib], ok[calib], target=0.95)
+    sub = ex[test]
+    curves_t, fb_t, sweep_t = sur.sweep(sub, eval_seeds, threshold=t_star, block=block)
+    etp, etf = relative_errors(curves_t, ref[test])
+    ok_t = (etp <= spec["max_relative_error"]) & (etf <= spec["max_relative_error"])
+    plain_rate = n_eval / plain_seconds
+    metrics["calibrated"] = {
+        "target_fraction_within": 0.95, "threshold": t_star, "calibration_points": int(len(calib)),
+        "test_points": int(len(test)), "test_fallback_fraction": float(fb_t.mean()),
+        "test_fraction_within": float(ok_t.mean()),
+        "test_median_rel_err_peak": float(np.median(etp)), "test_median_rel_err_final": float(np.median(etf)),
+        "test_points_per_s": len(test) / sweep_t, "speedup_vs_plain_sweep": (len(test) / sweep_t) / plain_rate,
+    }
+    np.savez_compressed(os.path.join(out_dir, "surrogate_eval_points.npz"), unit_x=ex, predicted_err=err_hat,
+                        rel_err_peak=ep_, rel_err_final=ef_, nn_rel_err_peak=np_, nn_rel_err_final=nf_)
     flat = [m.params() for m in sur.members]
     np.savez_compressed(os.path.join(out_dir, "surrogate_weights.npz"), member0=flat[0], member1=flat[1],
                         y_mean=sur.y_mean, y_std=sur.y_std)
