@@ -223,8 +223,8 @@ int upload_tables(epi_sim* s) {
 int enqueue_hour(epi_sim* s, uint32_t next_hour) {
   if (next_hour % 24u == 0) HIP_TRY(launch_hospital_scan(s->v, s->shape, s->stream));
   HIP_TRY(launch_step(s->v, s->shape, s->stream));
-  HIP_TRY(launch_epilogue(s->v, s->shape, s->stream));
   if (hour_has_commit(next_hour)) HIP_TRY(launch_commit(s->v, s->shape, next_hour, s->stream));
+  else HIP_TRY(launch_epilogue(s->v, s->shape, s->stream));
   return EPI_OK;
 }
 
@@ -979,10 +979,10 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
       }
       (void)hipEventRecord(e[1], sim->stream);
       if (launch_step(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
-      if (launch_epilogue(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       (void)hipEventRecord(e[2], sim->stream);
-      if (hour_has_commit(h) && launch_commit(sim->v, sim->shape, h, sim->stream) != hipSuccess)
-        rc = fail(EPI_ERR_DEVICE, "launch");
+      const hipError_t le = hour_has_commit(h) ? launch_commit(sim->v, sim->shape, h, sim->stream)
+                                               : launch_epilogue(sim->v, sim->shape, sim->stream);
+      if (le != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       (void)hipEventRecord(e[3], sim->stream);
       ++sim->host_hour;
     }

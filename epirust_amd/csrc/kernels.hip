@@ -12,8 +12,9 @@
 //              runs the intervention triggers and the stop rule on device.
 //   k_commit : per agent, the claim winner moves and grid bytes are updated. It reads
 //              nothing the same hour's epilogue writes (its hour comes from a slot the
-//              previous epilogue filled), so the two need not be ordered. Hours of day
-//              1..6 move nobody and launch no commit.
+//              previous epilogue filled), so the epilogue runs as the launch's first
+//              blocks, one per simulation, beside the commit. Hours of day 1..6 move
+//              nobody and launch k_epilogue alone.
 // A sweep decided at hour h (allocation_map.rs:306-337) is applied by k_step of hour h+1
 // before the routine, which is what the reference's next hour sees; sweeps are
 // idempotent, so k_apply_sweep re-applies the last one before the host reads agents.
@@ -431,57 +432,98 @@ struct Agent {
   }
 };
 
-__device__ __forceinline__ void add_event(const DevView& v, const SimDesc& D, SimState* S, uint32_t hour, uint32_t kind) {
-  const uint32_t n = S->n_events;
+// The descriptor fields the epilogue reads.
+struct EpiDesc {
+  uint64_t row_off, event_off;
+  uint32_t event_cap, hours, n_vacc, vacc_off;
+  uint32_t has_lockdown, lockdown_at, has_hospital, spread_rate_threshold;
+};
+
+__device__ __forceinline__ void add_event(const DevView& v, const EpiDesc& D, SimState& st, uint32_t hour,
+                                          uint32_t kind) {
+  const uint32_t n = st.n_events;
   if (n < D.event_cap) v.events[D.event_off + n] = Event{hour, kind};
-  S->n_events = n + 1;
+  st.n_events = n + 1;
 }
 
-// Hour epilogue for one simulation (k_epilogue, after k_step):
+// What the epilogue reads besides the counts, loaded by its thread 0 before the counts are
+// reduced so the loads overlap: the simulation's descriptor and state, and the next entry
+// of the vaccination schedule.
+struct EpilogueIn {
+  EpiDesc D;
+  SimState st;
+  uint32_t vacc_h;           // hour of schedule entry st.vacc_next, ~0u past the end
+  unsigned long long vacc_t;
+};
+
+__device__ __forceinline__ void load_epilogue_in(const DevView& v, uint32_t sim, EpilogueIn& in) {
+  const SimDesc& d = v.desc[sim];
+  in.D = EpiDesc{d.row_off, d.event_off, d.event_cap, d.hours, d.n_vacc, d.vacc_off,
+                 d.has_lockdown, d.lockdown_at, d.has_hospital, d.spread_rate_threshold};
+  in.st = v.state[sim];
+  const bool more = in.st.vacc_next < in.D.n_vacc;
+  in.vacc_h = more ? v.vacc_hour[in.D.vacc_off + in.st.vacc_next] : ~0u;
+  in.vacc_t = more ? v.vacc_thr[in.D.vacc_off + in.st.vacc_next] : 0ull;
+}
+
+// Hour epilogue for one simulation (after k_step):
 // the CSV row (counts.rs:126-140), interventions (allocation_map.rs:306-337,
 // lockdown.rs:55-73, hospital.rs:51-74, vaccination.rs:52-54) and the standalone stop
-// rule (epidemiology_simulation.rs:564-575).
-__device__ void hour_epilogue(const DevView& v, const SimDesc& D, SimState* S, uint32_t h, const uint32_t* c) {
+// rule (epidemiology_simulation.rs:564-575). Works on the registers of `in` and stores
+// the state once at the end.
+__device__ void hour_epilogue(const DevView& v, EpilogueIn& in, SimState* S, uint32_t h, const uint32_t* c) {
+  const EpiDesc& D = in.D;
+  SimState& st = in.st;
   v.rows[D.row_off + (h - 1)] = Counts7{h, c[0], c[1], c[2], c[3], c[4], c[5]};
   const uint32_t infected = c[2];
   uint32_t sweep = 0;
   if (D.n_vacc) {
     // schedule sorted by hour, one entry per hour (the last config entry wins); hours
     // advance by one per call, so a cursor replaces a search
-    uint32_t k = S->vacc_next;
-    while (k < D.n_vacc && v.vacc_hour[D.vacc_off + k] < h) ++k;
-    if (k < D.n_vacc && v.vacc_hour[D.vacc_off + k] == h) {
+    uint32_t k = st.vacc_next, vh = in.vacc_h;
+    unsigned long long vt = in.vacc_t;
+    while (k < D.n_vacc && vh < h) {
+      ++k;
+      if (k < D.n_vacc) {
+        vh = v.vacc_hour[D.vacc_off + k];
+        vt = v.vacc_thr[D.vacc_off + k];
+      }
+    }
+    if (k < D.n_vacc && vh == h) {
       sweep |= SWEEP_VACC;
-      S->vacc_thr = v.vacc_thr[D.vacc_off + k];
-      S->vacc_hour = h;
-      add_event(v, D, S, h, EV_VACCINATION);
+      st.vacc_thr = vt;
+      st.vacc_hour = h;
+      add_event(v, D, st, h, EV_VACCINATION);
       ++k;
     }
-    S->vacc_next = k;
+    st.vacc_next = k;
   }
-  if (D.has_lockdown && !S->locked && h % 24u == 0 && infected > D.lockdown_at) {
-    S->locked = 1;
+  if (D.has_lockdown && !st.locked && h % 24u == 0 && infected > D.lockdown_at) {
+    st.locked = 1;
     sweep |= SWEEP_LOCK;
-    add_event(v, D, S, h, EV_LOCKED);
+    add_event(v, D, st, h, EV_LOCKED);
   }
-  if (S->locked && h == S->zero_infection_hour + 21u * 24u) {
+  if (st.locked && h == st.zero_infection_hour + 21u * 24u) {
     sweep |= SWEEP_UNLOCK;
-    S->locked = 0;
-    S->zero_infection_hour = 0;
-    add_event(v, D, S, h, EV_UNLOCKED);
+    st.locked = 0;
+    st.zero_infection_hour = 0;
+    add_event(v, D, st, h, EV_UNLOCKED);
   }
-  if (h % 24u == 0) S->hosp_new_infections = infected > S->hosp_new_infections ? infected - S->hosp_new_infections : 0u;
-  if (!S->hosp_applied && h % 24u == 0 && D.has_hospital && S->hosp_new_infections >= D.spread_rate_threshold) {
-    S->hosp_version = 1;
-    S->hosp_applied = 1;
-    add_event(v, D, S, h, EV_HOSPITAL);
+  if (h % 24u == 0) st.hosp_new_infections = infected > st.hosp_new_infections ? infected - st.hosp_new_infections : 0u;
+  if (!st.hosp_applied && h % 24u == 0 && D.has_hospital && st.hosp_new_infections >= D.spread_rate_threshold) {
+    st.hosp_version = 1;
+    st.hosp_applied = 1;
+    add_event(v, D, st, h, EV_HOSPITAL);
   }
-  S->sweep = sweep;
-  if ((c[1] == 0 && c[2] == 0 && c[3] == 0) || h + 1 >= D.hours) S->active = 0;
-  S->hour = h;
-  S->rows_written = h;
-  S->first_vacant = ~0u;  // this hour's scan is consumed; the next one starts fresh
-  S->commit_hour[(h + 1u) & 1u] = S->active ? h + 1u : 0u;
+  st.sweep = sweep;
+  if ((c[1] == 0 && c[2] == 0 && c[3] == 0) || h + 1 >= D.hours) st.active = 0;
+  st.hour = h;
+  st.rows_written = h;
+  st.first_vacant = ~0u;  // this hour's scan is consumed; the next one starts fresh
+  const uint32_t next = st.active ? h + 1u : 0u;  // constant indices keep st in registers
+  if (h & 1u) st.commit_hour[0] = next;
+  else st.commit_hour[1] = next;
+  *S = st;
 }
 
 // The sweep decided at the previous hour (allocation_map.rs:349-387): vaccination of
@@ -500,17 +542,20 @@ __device__ __forceinline__ uint32_t apply_sweep(uint32_t w, uint32_t sweep, uint
 
 // Block-wide hour epilogue of one simulation: reduce the hour's per-block partial counts
 // (hours of day 2..6 repeat the previous row: nothing changed), then hour_epilogue on
-// thread 0. `h` must be read before any thread can reach hour_epilogue.
-__device__ void run_epilogue(const DevView& v, const SimDesc& D, SimState* S, uint32_t h) {
+// thread 0, whose other inputs are loaded first so they arrive with the partials.
+__device__ void run_epilogue(const DevView& v, uint32_t sim, const SimDesc& D, SimState* S, uint32_t h) {
   __shared__ uint32_t s_sum[6];
+  EpilogueIn in;
   if (h % 24u >= 2 && h % 24u <= 6) {
     if (threadIdx.x == 0) {
+      load_epilogue_in(v, sim, in);
       const Counts7 prev = v.rows[D.row_off + (h - 2)];
       const uint32_t c[6] = {prev.s, prev.e, prev.i, prev.h, prev.r, prev.d};
-      hour_epilogue(v, D, S, h, c);
+      hour_epilogue(v, in, S, h, c);
     }
     return;
   }
+  if (threadIdx.x == 0) load_epilogue_in(v, sim, in);
   if (threadIdx.x < 6) s_sum[threadIdx.x] = 0;
   __syncthreads();
   uint32_t acc[6] = {0, 0, 0, 0, 0, 0};
@@ -526,7 +571,7 @@ __device__ void run_epilogue(const DevView& v, const SimDesc& D, SimState* S, ui
     if ((threadIdx.x & (kWave - 1)) == 0 && x) atomicAdd(&s_sum[k], x);
   }
   __syncthreads();
-  if (threadIdx.x == 0) hour_epilogue(v, D, S, h, s_sum);
+  if (threadIdx.x == 0) hour_epilogue(v, in, S, h, s_sum);
 }
 
 __global__ void k_step(DevView v) {
@@ -627,20 +672,27 @@ __global__ void k_step(DevView v) {
   if (threadIdx.x < 6) v.block_counts[6u * blockIdx.x + threadIdx.x] = s_counts[threadIdx.x];
 }
 
-// One block per simulation: the hour's epilogue.
-__global__ void k_epilogue(DevView v) {
-  const uint32_t sim = blockIdx.x;
+// The hour's epilogue of simulation `sim` by one block. A stopped simulation only clears
+// the commit slot of the parity it last stepped (the slot of the next commit but one).
+__device__ __forceinline__ void epilogue_block(const DevView& v, uint32_t sim) {
   SimState* S = v.state + sim;
-  if (!S->active) {  // uniform; the slot of the commit two hours on must say "none"
+  if (!S->active) {  // uniform
     if (threadIdx.x == 0) S->commit_hour[S->hour & 1u] = 0u;
     return;
   }
-  run_epilogue(v, v.desc[sim], S, S->hour + 1);
+  run_epilogue(v, sim, v.desc[sim], S, S->hour + 1);
 }
 
-// `parity` is the hour's parity (an hour of day 24k + j has the parity of j).
-__global__ void k_commit(DevView v, uint32_t parity) {
-  const BlockRef bm = v.agent_blocks[blockIdx.x];
+__global__ void k_epilogue(DevView v) { epilogue_block(v, blockIdx.x); }
+
+// `parity` is the hour's parity (an hour of day 24k + j has the parity of j). Blocks
+// 0 .. n_epi-1 run the epilogue of simulation blockIdx.x; the rest commit agent blocks.
+__global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
+  if (blockIdx.x < n_epi) {  // uniform
+    epilogue_block(v, blockIdx.x);
+    return;
+  }
+  const BlockRef bm = v.agent_blocks[blockIdx.x - n_epi];
   const uint32_t sim = first_lane(bm.sim);
   const SimDesc& D = v.desc[sim];
   const SimState* S = v.state + sim;
@@ -747,8 +799,8 @@ hipError_t launch_epilogue(const DevView& v, const LaunchShape& s, hipStream_t s
 }
 
 hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, hipStream_t stream) {
-  if (s.agent_blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_commit, dim3(s.agent_blocks), dim3(s.block), 0, stream, v, hour & 1u);
+  if (s.agent_blocks + s.n_sims == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_commit, dim3(s.n_sims + s.agent_blocks), dim3(s.block), 0, stream, v, hour & 1u, s.n_sims);
   return hipGetLastError();
 }
 

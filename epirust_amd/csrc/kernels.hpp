@@ -37,8 +37,9 @@ struct LaunchShape {
   uint32_t n_sims = 0;
 };
 
-// One hour: [hour of day 0] scan, step, epilogue, then commit (hours of day 0 and 7..23;
-// `hour` only needs the right parity, so a captured day of hours 1..24 replays for any day).
+// One hour: [hour of day 0] scan, step, then either the commit with the epilogue in its
+// first blocks (hours of day 0 and 7..23) or the epilogue alone (hours of day 1..6).
+// `hour` only needs the right parity, so a captured day of hours 1..24 replays for any day.
 hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStream_t stream);
 hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream);
 hipError_t launch_epilogue(const DevView& v, const LaunchShape& s, hipStream_t stream);
