@@ -110,12 +110,12 @@ def traffic(fetch_csv, write_csv, agents, grid, hours, command):
     per_kernel = collections.defaultdict(lambda: [0.0, 0.0, 0])
     for d, v in fetch.items():
         k = short(names[d])
-        if k.startswith("k_") and k != "k_fill_u64":
+        if k in HOUR_KERNELS:
             per_kernel[k][0] += v * f_scale * 1024
             per_kernel[k][2] += 1
     for d, v in write.items():
         k = short(names_w[d])
-        if k.startswith("k_") and k != "k_fill_u64":
+        if k in HOUR_KERNELS:
             per_kernel[k][1] += v * w_scale * 1024
     hour_bytes = sum(f + w for f, w, _ in per_kernel.values()) / hours
     algo = 32.0 * agents + 8.0 * grid * grid
