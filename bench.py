@@ -37,7 +37,7 @@ def algorithmic_bytes_per_hour(n, g):
 
 
 def cpu_threads():
-    # the GPU box's CPU share for one GPU is 16 cores
+    # a one-GPU host's CPU share is 16 cores; use at most that many oracle threads
     return max(1, min(16, os.cpu_count() or 1))
 
 
@@ -232,8 +232,8 @@ def main():
                          "calibration dispatch for FETCH_SIZE / WRITE_SIZE, and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (one region per GPU; "
-              "DESIGN.md describes how the step would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path; the step would shard as one "
+              "region per GPU with halo and claim exchange over RCCL.", file=sys.stderr)
         sys.exit(2)
 
     import epirust_amd as G

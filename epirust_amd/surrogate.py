@@ -4,7 +4,7 @@ An MLP (9 -> 4 x 512 GELU -> 100 days x 6 compartments) maps transmission, mobil
 intervention parameters to the daily S/E/I/H/R/D fractions of a 100,000-agent region
 (632 x 632 grid, 2,400 hours). It is trained on ensembles that the batched GPU simulator
 generates, with the library's own bf16-MFMA forward/backward kernels (epi_mlp_*), and the
-plain simulation stays the fallback and the judge of its error:
+plain simulation stays the fallback and the measure of its error:
 
 * inputs are Sobol points over the ranges in tests/golden/configs/cfg5_surrogate.json;
 * a daily sample is the row with hour % 24 == 1 (the reference's plotting rule,
@@ -390,7 +390,7 @@ def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=60,
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default="gpurun_out/surrogate")
+    ap.add_argument("--out", default="surrogate_out")
     ap.add_argument("--train", type=int, default=0)
     ap.add_argument("--eval", type=int, default=0)
     ap.add_argument("--seeds", type=int, default=0)
