@@ -36,6 +36,15 @@ def algorithmic_bytes_per_hour(n, g):
     return 32.0 * n + 8.0 * g * g
 
 
+def measured_traffic(name):
+    """HBM bytes per simulated hour from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_hour")
+
+
 def cpu_threads():
     # a one-GPU host's CPU share is 16 cores; use at most that many oracle threads
     return max(1, min(16, os.cpu_count() or 1))
@@ -163,7 +172,13 @@ def bench_interventions(G, cpu_seconds):
     ev = s.events()
     s.close()
     n = cfg["population"]["Auto"]["number_of_agents"]
+    g = cfg["geography_parameters"]["grid_size"]
+    achieved = algorithmic_bytes_per_hour(n, g) * len(rows) / dt / 1e9
     out = {"metric": "agent-hours/s @10M agents with interventions", "value": n * len(rows) / dt,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_hour": algorithmic_bytes_per_hour(n, g),
+                        "basis": "wall time of the whole run (all hour kernels and graph launches)",
+                        "traffic": measured_traffic("traffic_cfg3_10m.json")},
            "unit": "agent-hours/s", "hours": len(rows), "seconds": dt,
            "events": {"vaccination": sum(1 for e in ev if e[1] == 0), "lockdown": sum(1 for e in ev if e[1] in (1, 2)),
                       "build_new_hospital": sum(1 for e in ev if e[1] == 3)},
@@ -249,11 +264,7 @@ def main():
     commit_ms, _ = prof["commit"]
     hour_ms = step_ms + commit_ms + (scan_ms * n_scan / max(n_step, 1))
     achieved = algorithmic_bytes_per_hour(n, g) / (hour_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_cfg2_1m.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_hour")
+    traffic = measured_traffic("traffic_cfg2_1m.json")
     result = {
         "metric": "agent-hours/s @1M agents",
         "value": value,

@@ -210,3 +210,29 @@ def test_csv_population_bit_exact(cfg1, tmp_path):
     cfg["disease"]["regular_transmission_rate"] = 0.2
     cfg["disease"]["high_transmission_rate"] = 0.5
     run_lockstep(cfg, 400, "csv population", every=8)
+
+
+def test_sweeps_at_sleep_hours_and_at_the_last_hour(cfg1):
+    """Sweeps are applied by the next hour's step (and re-applied before read-back): three
+    vaccinations on consecutive sleep hours (3, 4, 5 of day 2), a lockdown, and one
+    vaccination at the very last hour. Compared with the oracle both through one
+    graph-replayed run and hour by hour."""
+    cfg = interventions_config(cfg1)
+    cfg["hours"] = 600
+    cfg["interventions"] = [{"Lockdown": {"at_number_of_infections": 20, "essential_workers_population": 0.3}},
+                            {"Vaccinate": {"at_hour": 51, "percent": 0.2}},
+                            {"Vaccinate": {"at_hour": 52, "percent": 0.3}},
+                            {"Vaccinate": {"at_hour": 53, "percent": 0.1}},
+                            {"Vaccinate": {"at_hour": 599, "percent": 0.5}}]
+    say("[parity] sweeps: whole run")
+    gpu = G.Simulation(cfg)
+    cpu = O.OracleSim(cfg)
+    g_rows, g_stop = gpu.step(599)
+    c_rows, c_stop = cpu.step(599)
+    assert np.array_equal(g_rows, c_rows) and g_stop == c_stop
+    assert_agents_equal(gpu.agents(), cpu.agents(), "sweeps whole run")
+    assert gpu.events() == cpu.events()
+    gpu.close()
+    cpu.close()
+    h = run_lockstep(cfg, 80, "sweeps hourly", every=1)
+    assert h == 80
