@@ -36,6 +36,11 @@ def algorithmic_bytes_per_hour(n, g):
     return 32.0 * n + 8.0 * g * g
 
 
+def tuned_block_of(cfg):
+    from epirust_amd import autotune
+    return autotune.tuned_block([cfg])
+
+
 def measured_traffic(name):
     """HBM bytes per simulated hour from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
     path = os.path.join(ROOT, "profiles", name)
@@ -109,10 +114,13 @@ def bench_headline(args, G):
     g = cfg["geography_parameters"]["grid_size"]
     run_hours = cfg["hours"] - 1
 
+    from epirust_amd import autotune
+    block = args.block or autotune.tuned_block([cfg])
+
     def make():
         s = G.Simulation(cfg)
-        if args.block:
-            s.set_block_size(args.block)
+        if block:
+            s.set_block_size(block)
         return s
 
     if args.warmup > 0:
@@ -167,7 +175,11 @@ def bench_replicates(G, cpu_seconds):
 def bench_interventions(G, cpu_seconds):
     with open(os.path.join(CONFIGS, "cfg3_10m_interventions.json")) as f:
         cfg = json.load(f)
+    from epirust_amd import autotune
     s = G.Simulation(cfg)
+    block = autotune.tuned_block([cfg])
+    if block:
+        s.set_block_size(block)
     dt, rows = timed_steps(s, cfg["hours"] - 1)
     ev = s.events()
     s.close()
@@ -281,6 +293,7 @@ def main():
         "config": {"workload": "EpiRust standalone hour step, %d agents, %dx%d grid, %d hours, seed %s, "
                                "deterministic mode" % (n, g, g, run_hours, cfg.get("seed", 0)),
                    "agents": n, "grid_size": g, "hours": run_hours,
+                   "threads_per_block": args.block or tuned_block_of(cfg) or 256,
                    "kernel_ms": {"hospital_scan": scan_ms, "step": step_ms, "commit_or_epilogue": commit_ms}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
