@@ -305,6 +305,10 @@ def main():
         rate, sample = oracle_rate([cfg], cpu_s)
         result["cpu_baseline"] = {"value": rate, "unit": "agent-hours/s", "cores": cpu_threads(), "kind": "port",
                                   "sample": sample}
+        # the reference engine's default rayon pool is 4 threads (engine-app/src/main.rs:80-82)
+        rate4, sample4 = oracle_rate([cfg], cpu_s / 3, threads=4)
+        result["cpu_baseline_4_threads"] = {"value": rate4, "unit": "agent-hours/s", "cores": 4, "kind": "port",
+                                            "sample": sample4}
     if not args.skip_extra:
         result["replicates"] = bench_replicates(G, cpu_s / 3)
         result["interventions_10m"] = bench_interventions(G, cpu_s / 3)
