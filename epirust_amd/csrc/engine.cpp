@@ -191,7 +191,7 @@ int validate(const Config& c) {
       return fail(EPI_ERR_PANIC, "called `Result::unwrap()` on an `Err` value: InvalidProbability (Vaccinate.percent=" +
                                      std::to_string(v.percent) + ")");
   if (c.hours > 24u * kMaxDay) return fail(EPI_ERR_CONFIG, "hours above the supported maximum of 98280");
-  if (c.number_of_agents > (1u << 30)) return fail(EPI_ERR_CONFIG, "number_of_agents above 2^30");
+  if (c.number_of_agents > kMaxAgents) return fail(EPI_ERR_CONFIG, "number_of_agents above 2^27 - 1");
   return EPI_OK;
 }
 
@@ -603,7 +603,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     HIP_TRY(hipMemcpy(d_vacc_thr, s->vacc_thr.data(), s->vacc_thr.size() * sizeof(unsigned long long),
                       hipMemcpyHostToDevice));
   }
-  HIP_TRY(launch_fill_u64(v.claim, s->n_cells, ~0ull, s->stream));
+  HIP_TRY(launch_fill_u32(v.claim, s->n_cells, ~0u, s->stream));
   rc = upload_tables(s.get());
   if (rc != EPI_OK) return rc;
   HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_active), 2 * s->n_sims * sizeof(uint32_t),

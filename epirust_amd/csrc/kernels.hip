@@ -5,9 +5,9 @@
 //   [hour-of-day 0 only] k_hospital_scan : first vacant hospital cell of the snapshot
 //   k_step   : per agent, the previous hour's vaccination / lockdown sweep, then routine +
 //              disease update against the start-of-hour grid; state written in place (it
-//              is read by no other agent), the chosen cell claimed with a 64-bit atomicMin
-//              keyed (hour, agent id); S/E/I/H/R/D counted with wavefront ballots into
-//              per-block partials.
+//              is read by no other agent), the chosen cell claimed with a 32-bit atomicMin
+//              keyed (claim hour, agent id); S/E/I/H/R/D counted with wavefront ballots
+//              into per-block partials. At hour of day 2 it resets the claim array.
 //   k_epilogue: one block per simulation reduces the partials, writes the hour's row and
 //              runs the intervention triggers and the stop rule on device.
 //   k_commit : per agent, the claim winner moves and grid bytes are updated. It reads
@@ -83,8 +83,24 @@ __device__ __forceinline__ uint64_t cell_index(const SimDesc& D, int32_t x, int3
   return static_cast<uint64_t>(static_cast<uint32_t>(y) + kPadY) * D.pitch + static_cast<uint32_t>(x) + kPadX;
 }
 
-__device__ __forceinline__ unsigned long long claim_key(uint32_t hour, uint32_t id) {
-  return (static_cast<unsigned long long>(0xFFFFFFFFu - hour) << 32) | id;
+// Claim key of agent `id` at `hour`: hours of day 7..23, 0 (the hours that move anyone)
+// rank 0..17 since the reset at hour of day 2; a later hour takes a smaller top field, so
+// its claims undercut any left from earlier hours of the day, and within an hour the
+// lowest id wins.
+__device__ __forceinline__ uint32_t claim_key(uint32_t hour, uint32_t id) {
+  const uint32_t rank = (hour % 24u + 17u) % 24u;
+  return ((31u - rank) << kClaimIdBits) | id;
+}
+
+// Hour of day 2: block b of a simulation's n_blocks resets its share of the claim array
+// (all padded cells, in 16-byte stores; the array of each simulation is a multiple of 4
+// cells long and starts at a multiple of 4).
+__device__ __forceinline__ void reset_claims(const DevView& v, const SimDesc& D, uint32_t b) {
+  const uint32_t quads = D.pitch / 4u * (D.height + 2u * kPadY);
+  const uint32_t per = (quads + D.n_blocks - 1u) / D.n_blocks;
+  const uint32_t end = min(quads, (b + 1u) * per);
+  uint4* q = reinterpret_cast<uint4*>(v.claim + D.cell_off);
+  for (uint32_t k = b * per + threadIdx.x; k < end; k += blockDim.x) q[k] = make_uint4(~0u, ~0u, ~0u, ~0u);
 }
 
 // Per-thread state of one agent's hour.
@@ -589,6 +605,7 @@ __global__ void k_step(DevView v) {
   // home, and no movement or disease update runs while asleep (citizen/mod.rs:244-248).
   // k_epilogue repeats the previous hour's counts for them; only a pending sweep is applied.
   if (hd >= 2 && hd <= 6) {
+    if (hd == 2) reset_claims(v, D, blockIdx.x - D.block_begin);
     if (sweep && valid) {
       const uint64_t gi = D.agent_off + local;
       const uint32_t w = v.st[gi];
@@ -772,7 +789,7 @@ __global__ void k_hospital_scan(DevView v, uint32_t cells_per_thread) {
   if ((threadIdx.x & (kWave - 1)) == 0 && best != ~0u) atomicMin(&S->first_vacant, best);
 }
 
-__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long value) {
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t k = i; k < n; k += stride) p[k] = value;
@@ -810,10 +827,10 @@ hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long value, hipStream_t stream) {
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   const uint64_t blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(k_fill_u64, dim3(static_cast<uint32_t>(blocks < 65536 ? blocks : 65536)), dim3(256), 0, stream, p,
+  hipLaunchKernelGGL(k_fill_u32, dim3(static_cast<uint32_t>(blocks < 65536 ? blocks : 65536)), dim3(256), 0, stream, p,
                      n, value);
   return hipGetLastError();
 }

@@ -16,7 +16,7 @@ struct DevView {
   uint32_t* wsa;
   uint32_t* tgt;
   uint8_t* occ;
-  unsigned long long* claim;
+  uint32_t* claim;  // per cell: lowest claim key of the day's latest claim hour, ~0u = none
   const SimDesc* desc;
   SimState* state;
   const BlockRef* agent_blocks;  // per agent block: (sim, first local agent)
@@ -47,6 +47,6 @@ hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, 
 // Re-applies the last decided sweep of every simulation (idempotent), before agent read-back.
 hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_t stream);
 inline bool hour_has_commit(uint32_t hour) { return hour % 24u == 0 || hour % 24u >= 7; }
-hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long value, hipStream_t stream);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t stream);
 
 }  // namespace epi

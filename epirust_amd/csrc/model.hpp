@@ -13,7 +13,10 @@
 // (x, y) lives at (y + kPadY) * pitch + x + kPadX: a zero border of kPadX columns / kPadY
 // rows (and pitch >= stride + 9, a multiple of 4) lets every kernel read the 5x5
 // neighbourhood of any cell as ten aligned dwords without bounds checks. Claims are a
-// parallel u64 array with the same indexing, keyed by (hour, agent id).
+// parallel u32 array with the same indexing: a claim key is the rank of the claim hour
+// within the day (hours of day 7..23 then 0, later hours smaller) in the top kClaimHourBits
+// bits over the agent id, and the array is reset at hour of day 2, before the day's first
+// claim hour, so a stale claim never ranks with the current hour's.
 #pragma once
 #include <cstdint>
 
@@ -67,6 +70,8 @@ struct Rect {
 };
 
 constexpr uint32_t kPadX = 4, kPadY = 2;
+constexpr uint32_t kClaimHourBits = 5, kClaimIdBits = 32 - kClaimHourBits;
+constexpr uint32_t kMaxAgents = (1u << kClaimIdBits) - 1u;  // agent ids fit the claim key
 inline uint32_t padded_pitch(uint32_t stride) { return (stride + 9u + 3u) & ~3u; }
 inline uint64_t padded_cells(uint32_t stride, uint32_t height) {
   return static_cast<uint64_t>(padded_pitch(stride)) * (height + 2u * kPadY);
