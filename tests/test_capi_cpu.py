@@ -65,6 +65,11 @@ def test_config_errors_are_reported_not_aborted(cfg1):
     c["disease"]["death_rate"] = 1.5
     rc, msg, _ = _create(json.dumps(c))
     assert rc == 2 and "InvalidProbability" in msg
+    # agent ids share a 32-bit claim key with a 5-bit hour rank
+    c = json.loads(json.dumps(cfg1))
+    c["population"]["Auto"]["number_of_agents"] = 1 << 27
+    rc, msg, _ = _create(json.dumps(c))
+    assert rc == 1 and msg == "number_of_agents above 2^27 - 1"
 
 
 def test_unknown_keys_are_ignored_like_serde(cfg1):
