@@ -30,8 +30,23 @@ namespace epi {
 namespace {
 
 constexpr int kWave = 64;
+// Agents of one simulation that the chip holds at once (256 CUs x 2048 threads). Blocks
+// are dispatched in agent order, so agents below this id run in the first residency round,
+// concurrently with every lower id: a claim pre-check there finds no earlier claims and
+// only adds a load.
+constexpr uint32_t kResidentAgents = 256u * 2048u;
 
 __device__ __forceinline__ uint32_t first_lane(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Makes the values pass through registers at this point: the loads producing them are
+// issued before it and waited for once, together. Without it the compiler sinks each load
+// to its first use, behind the wait for another, and a lane pays one memory round trip per
+// word instead of one for all of them. The asm has outputs and no side effects, so it is
+// no memory clobber: descriptor reads after it still come from the scalar cache.
+__device__ __forceinline__ void loaded_together(uint32_t& a, uint32_t& b, uint32_t& c) {
+  asm("" : "+v"(a), "+v"(b), "+v"(c));
+}
+__device__ __forceinline__ void loaded_together(uint32_t& a, uint32_t& b) { asm("" : "+v"(a), "+v"(b)); }
 
 __device__ __forceinline__ bool contains(const Rect& a, int32_t x, int32_t y) {
   return a.x0 <= x && a.x1 >= x && a.y0 <= y && a.y1 >= y;
@@ -94,13 +109,19 @@ __device__ __forceinline__ uint32_t claim_key(uint32_t hour, uint32_t id) {
 
 // Hour of day 2: block b of a simulation's n_blocks resets its share of the claim array
 // (all padded cells, in 16-byte stores; the array of each simulation is a multiple of 4
-// cells long and starts at a multiple of 4).
-__device__ __forceinline__ void reset_claims(const DevView& v, const SimDesc& D, uint32_t b) {
+// cells long and starts at a multiple of 4). The share is computed before the block's
+// first store, while the descriptor still comes from the scalar cache.
+struct ClaimShare {
+  uint4* q;
+  uint32_t begin, end;
+};
+__device__ __forceinline__ ClaimShare claim_share(const DevView& v, const SimDesc& D, uint32_t b) {
   const uint32_t quads = D.pitch / 4u * (D.height + 2u * kPadY);
   const uint32_t per = (quads + D.n_blocks - 1u) / D.n_blocks;
-  const uint32_t end = min(quads, (b + 1u) * per);
-  uint4* q = reinterpret_cast<uint4*>(v.claim + D.cell_off);
-  for (uint32_t k = b * per + threadIdx.x; k < end; k += blockDim.x) q[k] = make_uint4(~0u, ~0u, ~0u, ~0u);
+  return ClaimShare{reinterpret_cast<uint4*>(v.claim + D.cell_off), b * per, min(quads, (b + 1u) * per)};
+}
+__device__ __forceinline__ void reset_claims(const ClaimShare& c) {
+  for (uint32_t k = c.begin + threadIdx.x; k < c.end; k += blockDim.x) c.q[k] = make_uint4(~0u, ~0u, ~0u, ~0u);
 }
 
 // Per-thread state of one agent's hour.
@@ -108,7 +129,19 @@ struct Agent {
   const SimDesc& D;
   const SimState& S;
   const uint8_t* __restrict__ occ;  // this simulation's grid
+  // Transmission thresholds, read once up front: a per-lane choice between two descriptor
+  // fields would otherwise become a per-lane load of the chosen field.
+  const uint64_t thr_regular, thr_high;
   uint32_t id, h, w, tm, wsa;
+  const uint32_t* tm_arr = nullptr;  // this agent's stored tm word
+  const uint32_t* claims = nullptr;  // this simulation's claim words
+  // Claim word of the chosen cell as read right after a local move is chosen (~0u when not
+  // read). Claim words only decrease within an hour and every word left from earlier hours
+  // (or the daily reset) exceeds any of this hour's keys, so a value below this agent's key,
+  // however stale, means a lower id has claimed the cell: the agent has lost and skips its
+  // claim. Contended cells are mostly claimed first by lower ids in earlier blocks, so for
+  // agents past the first residency round this removes most same-address atomics.
+  uint32_t seen_claim = ~0u;
   int32_t x, y;                     // current cell
   int32_t tx, ty;                   // chosen cell
   int32_t hx = 0, hy = 0;           // home house origin (house id % H)
@@ -122,7 +155,8 @@ struct Agent {
   int32_t wx = 0, wy = 0, woff = 0;
   uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;  // named, so no variable-index access exists
 
-  __device__ Agent(const SimDesc& d, const SimState& s, const uint8_t* o) : D(d), S(s), occ(o) {}
+  __device__ Agent(const SimDesc& d, const SimState& s, const uint8_t* o)
+      : D(d), S(s), occ(o), thr_regular(d.thr_regular), thr_high(d.thr_high) {}
 
   __device__ __forceinline__ u32x4 dr(uint32_t site, uint32_t block) const {
     return philox4x32_10(id, h, site, block, D.seed_lo, D.seed_hi);
@@ -268,6 +302,7 @@ struct Agent {
     const int k = __builtin_ctz(cand);
     tx = lx + moore_dx(k);
     ty = ly + moore_dy(k);
+    if (id >= kResidentAgents) seen_claim = claims[cell_index(D, tx, ty)];  // overlaps the disease update
   }
 
   // default_disease_handler.rs:64-86 (OR over fixed per-neighbour draws)
@@ -306,14 +341,14 @@ struct Agent {
         q0 = q.w[0]; q1 = q.w[1]; q2 = q.w[2]; q3 = q.w[3];
       }
       const uint64_t v = (k & 1) ? join64(q2, q3) : join64(q0, q1);
-      const uint64_t thr = ((high >> k) & 1u) ? D.thr_high : D.thr_regular;
+      const uint64_t thr = ((high >> k) & 1u) ? thr_high : thr_regular;
       if (bernoulli(thr, v)) return true;
     }
     return false;
   }
 
   // disease_state_machine.rs:53-70 + default_disease_handler.rs:41-62
-  __device__ __forceinline__ void update_infection_dynamics(const uint32_t* __restrict__ tm_arr, uint64_t gi) {
+  __device__ __forceinline__ void update_infection_dynamics() {
     const uint32_t s = state();
     if (s == ST_S) {
       if (on_susceptible()) {
@@ -321,7 +356,7 @@ struct Agent {
         tm = h;
       }
     } else if (s == ST_E) {
-      tm = tm_arr[gi];
+      tm = *tm_arr;
       const u32x4& d = disease_words();
       const int32_t rf = static_cast<int32_t>(below(d.w[0], 3)) - 1;  // RANGE_FOR_EXPOSED
       if (h - tm >= static_cast<uint32_t>(static_cast<int32_t>(D.exposed_duration) + rf)) {
@@ -332,7 +367,7 @@ struct Agent {
         tm = symptoms ? h : 0u;
       }
     } else if (s == ST_I && sev() == SEV_PRE) {
-      tm = tm_arr[gi];
+      tm = *tm_arr;
       if (h - tm >= D.pre_symptomatic_duration) {
         const u32x4& d = disease_words();
         w = set_fld(w, B_SEV, M_SEV, bernoulli(D.thr_severe, join64(d.w[2], d.w[3])) ? SEV_SEVERE : SEV_MILD);
@@ -343,7 +378,7 @@ struct Agent {
 
   // citizen/mod.rs:257-349: choose the primitive and the area it leads to, run it, then the
   // disease update (skipped by the hospital-staff quarantine transitions, which return early).
-  __device__ __forceinline__ void perform_movements(uint32_t hd, const uint32_t* __restrict__ tm_arr, uint64_t gi) {
+  __device__ __forceinline__ void perform_movements(uint32_t hd) {
     const uint32_t s = ws();
     uint32_t act = ACT_NONE, new_area = ~0u;
     Rect target = D.housing;
@@ -400,7 +435,7 @@ struct Agent {
     }
     move(act, target);
     if (new_area != ~0u) set_area(new_area);
-    if (disease) update_infection_dynamics(tm_arr, gi);
+    if (disease) update_infection_dynamics();
   }
 
   // citizen/mod.rs:351-365 with allocation_map.rs:144-150
@@ -605,13 +640,15 @@ __global__ void k_step(DevView v) {
   // home, and no movement or disease update runs while asleep (citizen/mod.rs:244-248).
   // k_epilogue repeats the previous hour's counts for them; only a pending sweep is applied.
   if (hd >= 2 && hd <= 6) {
-    if (hd == 2) reset_claims(v, D, blockIdx.x - D.block_begin);
+    ClaimShare share{nullptr, 0, 0};
+    if (hd == 2) share = claim_share(v, D, blockIdx.x - D.block_begin);
     if (sweep && valid) {
       const uint64_t gi = D.agent_off + local;
       const uint32_t w = v.st[gi];
       const uint32_t nw = apply_sweep(w, sweep, local, *S, D);
       if (nw != w) v.st[gi] = nw;
     }
+    reset_claims(share);
     return;
   }
   const uint8_t* occ = v.occ + D.cell_off;
@@ -619,18 +656,24 @@ __global__ void k_step(DevView v) {
   uint32_t cat = 6;
   if (valid) {
     const uint64_t gi = D.agent_off + local;
+    // st and pos are loaded together, then the work that needs neither, then one wait
+    // (wsa and tm are read by few agents and loaded where they are needed).
+    uint32_t w_stored = v.st[gi];
+    uint32_t p = v.pos[gi];
     Agent a(D, *S, occ);
     a.id = local;
     a.h = h;
     a.locate_buildings();
-    const uint32_t w_stored = v.st[gi];
+    if (hd == 0 || hd >= 7) a.mv = a.dr(SITE_MOVE, 0);
+    loaded_together(w_stored, p);
     a.w = sweep ? apply_sweep(w_stored, sweep, local, *S, D) : w_stored;
-    const uint32_t p = v.pos[gi];
     a.x = static_cast<int32_t>(p & 0xFFFFu);
     a.y = static_cast<int32_t>(p >> 16);
     a.tx = a.x;
     a.ty = a.y;
     a.tm = 0;
+    a.tm_arr = v.tm + gi;
+    a.claims = v.claim + D.cell_off;
     const uint32_t w0 = w_stored;
     const bool staff = a.ws() == WS_STAFF;
     a.wsa = staff ? v.wsa[gi] : 0u;
@@ -639,7 +682,6 @@ __global__ void k_step(DevView v) {
 
     {
       const uint32_t s0 = a.state(), v0 = a.sev();
-      if (hd == 0 || hd >= 7) a.mv = a.dr(SITE_MOVE, 0);
       if (hd >= 7 && (s0 == ST_E || (s0 == ST_I && (v0 == SEV_PRE || (hd == 23 && v0 == SEV_SEVERE)))))
         a.dz = a.dr(SITE_DISEASE, 0);
     }
@@ -657,21 +699,25 @@ __global__ void k_step(DevView v) {
       tm_dirty = a.state() != s0;
     } else {
       const uint32_t s0 = a.state(), sv0 = a.sev();
-      a.perform_movements(hd, v.tm, gi);
+      a.perform_movements(hd);
       tm_dirty = a.state() != s0 || a.sev() != sv0;
     }
-    if (a.tx != a.x || a.ty != a.y) {
-      a.w |= F_MOVE;
-      v.tgt[gi] = static_cast<uint32_t>(a.tx) | (static_cast<uint32_t>(a.ty) << 16);
-      const uint64_t ci = D.cell_off + cell_index(D, a.tx, a.ty);
-      atomicMin(&v.claim[ci], claim_key(h, local));
-    }
+    // Everything the stores need is computed first: a descriptor field read after a store
+    // can no longer come from the scalar cache and costs a vector round trip.
+    const uint32_t key = claim_key(h, local);
+    const bool moves = (a.tx != a.x || a.ty != a.y) && key < a.seen_claim;
+    if (moves) a.w |= F_MOVE;
     if (grid_byte(D, a.w) != grid_byte(D, w0)) a.w |= F_DIRTY;
+    const uint64_t ci = D.cell_off + cell_index(D, a.tx, a.ty);
+    const uint32_t s = a.state();
+    cat = s == ST_S ? 0u : s == ST_E ? 1u : s == ST_I ? ((a.w & F_HOSP) ? 3u : 2u) : s == ST_R ? 4u : 5u;
+    if (moves) {
+      atomicMin(&v.claim[ci], key);
+      v.tgt[gi] = static_cast<uint32_t>(a.tx) | (static_cast<uint32_t>(a.ty) << 16);
+    }
     if (a.w != w0) v.st[gi] = a.w;
     if (tm_dirty) v.tm[gi] = a.tm;
     if (staff && a.wsa != wsa0) v.wsa[gi] = a.wsa;
-    const uint32_t s = a.state();
-    cat = s == ST_S ? 0u : s == ST_E ? 1u : s == ST_I ? ((a.w & F_HOSP) ? 3u : 2u) : s == ST_R ? 4u : 5u;
   }
 
   // S/E/I/H/R/D via wavefront ballots into LDS, then one plain store per block and
@@ -720,9 +766,10 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
     const uint64_t gi = D.agent_off + local;
     // st, pos and tgt are loaded together so a mover pays two dependent round trips
     // (these three, then its claim), not four.
-    const uint32_t w = v.st[gi];
-    const uint32_t p = v.pos[gi];
-    const uint32_t t = v.tgt[gi];
+    uint32_t w = v.st[gi];
+    uint32_t p = v.pos[gi];
+    uint32_t t = v.tgt[gi];
+    loaded_together(w, p, t);
     if (w & (F_MOVE | F_DIRTY)) {
       uint8_t* occ = v.occ + D.cell_off;
       const uint64_t old_ci = cell_index(D, static_cast<int32_t>(p & 0xFFFFu), static_cast<int32_t>(p >> 16));
