@@ -222,9 +222,8 @@ int upload_tables(epi_sim* s) {
 
 int enqueue_hour(epi_sim* s, uint32_t next_hour) {
   if (next_hour % 24u == 0) HIP_TRY(launch_hospital_scan(s->v, s->shape, s->stream));
-  HIP_TRY(launch_step(s->v, s->shape, s->stream));
-  if (hour_has_commit(next_hour)) HIP_TRY(launch_commit(s->v, s->shape, next_hour, s->stream));
-  else HIP_TRY(launch_epilogue(s->v, s->shape, s->stream));
+  if (hour_has_step(next_hour)) HIP_TRY(launch_step(s->v, s->shape, s->stream));
+  HIP_TRY(launch_commit(s->v, s->shape, next_hour, s->stream));
   return EPI_OK;
 }
 
@@ -569,7 +568,8 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     SimState st;
     std::memset(&st, 0, sizeof st);
     st.active = (s->cfg[i].hours > 1 && D.n > 0) ? 1u : 0u;
-    st.commit_hour[1] = st.active;  // hour 1 (odd); it moves nobody, but the slot must be right
+    st.commit_hour[1] = st.active;  // the agent pass of hour 1 (odd)
+    for (int k = 0; k < 6; ++k) st.last_counts[k] = pops[i].counts0[k + 1];  // hour 1 repeats them
     st.first_vacant = ~0u;
     s->state_h[i] = st;
     if (!host_init) {
@@ -968,7 +968,7 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
   int rc = EPI_OK;
   while (sim->host_hour < target && rc == EPI_OK) {
     const uint32_t nb = std::min(per_batch, target - sim->host_hour);
-    std::vector<int> had_scan(nb, 0);
+    std::vector<int> had_scan(nb, 0), had_step(nb, 0);
     for (uint32_t k = 0; k < nb && rc == EPI_OK; ++k) {
       const uint32_t h = sim->host_hour + 1;
       hipEvent_t* e = &ev[4 * k];
@@ -978,11 +978,10 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
         if (launch_hospital_scan(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       }
       (void)hipEventRecord(e[1], sim->stream);
-      if (launch_step(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      had_step[k] = hour_has_step(h) ? 1 : 0;
+      if (had_step[k] && launch_step(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       (void)hipEventRecord(e[2], sim->stream);
-      const hipError_t le = hour_has_commit(h) ? launch_commit(sim->v, sim->shape, h, sim->stream)
-                                               : launch_epilogue(sim->v, sim->shape, sim->stream);
-      if (le != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      if (launch_commit(sim->v, sim->shape, h, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       (void)hipEventRecord(e[3], sim->stream);
       ++sim->host_hour;
     }
@@ -994,7 +993,7 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
       (void)hipEventElapsedTime(&t[1], e[1], e[2]);
       (void)hipEventElapsedTime(&t[2], e[2], e[3]);
       if (had_scan[k]) { sum[0] += t[0]; ++cnt[0]; }
-      sum[1] += t[1]; ++cnt[1];
+      if (had_step[k]) { sum[1] += t[1]; ++cnt[1]; }
       sum[2] += t[2]; ++cnt[2];
     }
     ran += nb;

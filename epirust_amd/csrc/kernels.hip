@@ -3,21 +3,25 @@
 // One hour of EpiRust's standalone engine (engine/src/allocation_map.rs:67-134 and
 // :306-337, engine/src/citizen/mod.rs:227-454, default_disease_handler.rs:31-103) is
 //   [hour-of-day 0 only] k_hospital_scan : first vacant hospital cell of the snapshot
-//   k_step   : per agent, the previous hour's vaccination / lockdown sweep, then routine +
-//              disease update against the start-of-hour grid; state written in place (it
-//              is read by no other agent), the chosen cell claimed with a 32-bit atomicMin
-//              keyed (claim hour, agent id); S/E/I/H/R/D counted with wavefront ballots
-//              into per-block partials. At hour of day 2 it resets the claim array.
-//   k_epilogue: one block per simulation reduces the partials, writes the hour's row and
-//              runs the intervention triggers and the stop rule on device.
-//   k_commit : per agent, the claim winner moves and grid bytes are updated. It reads
-//              nothing the same hour's epilogue writes (its hour comes from a slot the
-//              previous epilogue filled), so the epilogue runs as the launch's first
-//              blocks, one per simulation, beside the commit. Hours of day 1..6 move
-//              nobody and launch k_epilogue alone.
-// A sweep decided at hour h (allocation_map.rs:306-337) is applied by k_step of hour h+1
-// before the routine, which is what the reference's next hour sees; sweeps are
-// idempotent, so k_apply_sweep re-applies the last one before the host reads agents.
+//   k_step   : [hours of day 0, 7..23] per agent, the previous hour's vaccination /
+//              lockdown sweep, then routine + disease update against the start-of-hour
+//              grid; state written in place (it is read by no other agent), the chosen
+//              cell claimed with a 32-bit atomicMin keyed (claim hour, agent id);
+//              S/E/I/H/R/D counted with wavefront ballots into per-block partials.
+//   k_commit : every hour. Its first blocks, one per simulation, run the hour's epilogue:
+//              reduce the partials, write the row, run the intervention triggers and the
+//              stop rule on device. The other blocks run the hour's agent pass, which
+//              reads nothing the same hour's epilogue writes (its hour and its sweep come
+//              from parity slots the previous epilogue filled), so the two share a launch.
+//              At hours of day 0 and 7..23 the agent pass commits the moves (the claim
+//              winner moves, grid bytes are updated). Hours of day 1..6 move nobody and
+//              change no state but flags and areas (citizen/mod.rs:244-248): the epilogue
+//              repeats the last counts and the agent pass applies the pending sweep, sets
+//              non-staff areas to home at hour of day 1 and resets the claim array at 2.
+// A sweep decided at hour h (allocation_map.rs:306-337) is applied by the agent pass of
+// hour h+1 (before the routine, in k_step), which is what the reference's next hour sees;
+// sweeps are idempotent, so k_apply_sweep re-applies the last one before the host reads
+// agents.
 // Every kernel is batched: blocks map to (simulation, agent range) through a table, so
 // many independent replicates share one launch. Nothing here uses MFMA: the step is
 // integer work bound by HBM traffic.
@@ -46,7 +50,11 @@ __device__ __forceinline__ uint32_t first_lane(uint32_t v) { return __builtin_am
 __device__ __forceinline__ void loaded_together(uint32_t& a, uint32_t& b, uint32_t& c) {
   asm("" : "+v"(a), "+v"(b), "+v"(c));
 }
-__device__ __forceinline__ void loaded_together(uint32_t& a, uint32_t& b) { asm("" : "+v"(a), "+v"(b)); }
+// The same for two loaded words and a draw block computed while they are in flight: the
+// draw's words pass through the barrier too, so its rounds are scheduled before the wait.
+__device__ __forceinline__ void loaded_together(uint32_t& a, uint32_t& b, u32x4& d) {
+  asm("" : "+v"(a), "+v"(b), "+v"(d.w[0]), "+v"(d.w[1]), "+v"(d.w[2]), "+v"(d.w[3]));
+}
 
 __device__ __forceinline__ bool contains(const Rect& a, int32_t x, int32_t y) {
   return a.x0 <= x && a.x1 >= x && a.y0 <= y && a.y1 >= y;
@@ -528,6 +536,7 @@ __device__ void hour_epilogue(const DevView& v, EpilogueIn& in, SimState* S, uin
   v.rows[D.row_off + (h - 1)] = Counts7{h, c[0], c[1], c[2], c[3], c[4], c[5]};
   const uint32_t infected = c[2];
   uint32_t sweep = 0;
+  unsigned long long vacc_thr = 0;
   if (D.n_vacc) {
     // schedule sorted by hour, one entry per hour (the last config entry wins); hours
     // advance by one per call, so a cursor replaces a search
@@ -542,8 +551,7 @@ __device__ void hour_epilogue(const DevView& v, EpilogueIn& in, SimState* S, uin
     }
     if (k < D.n_vacc && vh == h) {
       sweep |= SWEEP_VACC;
-      st.vacc_thr = vt;
-      st.vacc_hour = h;
+      vacc_thr = vt;
       add_event(v, D, st, h, EV_VACCINATION);
       ++k;
     }
@@ -566,25 +574,45 @@ __device__ void hour_epilogue(const DevView& v, EpilogueIn& in, SimState* S, uin
     st.hosp_applied = 1;
     add_event(v, D, st, h, EV_HOSPITAL);
   }
-  st.sweep = sweep;
   if ((c[1] == 0 && c[2] == 0 && c[3] == 0) || h + 1 >= D.hours) st.active = 0;
   st.hour = h;
   st.rows_written = h;
   st.first_vacant = ~0u;  // this hour's scan is consumed; the next one starts fresh
-  const uint32_t next = st.active ? h + 1u : 0u;  // constant indices keep st in registers
-  if (h & 1u) st.commit_hour[0] = next;
-  else st.commit_hour[1] = next;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) st.last_counts[k] = c[k];
+  // Slots of this hour's parity: the sweep (read by the next hour's agent pass) and the
+  // next hour's agent pass (0 once stopped). Constant indices keep st in registers.
+  const uint32_t next = st.active ? h + 1u : 0u;
+  if (h & 1u) {
+    st.sweep[1] = sweep;
+    st.sweep_vacc_hour[1] = h;
+    st.sweep_vacc_thr[1] = vacc_thr;
+    st.commit_hour[0] = next;
+  } else {
+    st.sweep[0] = sweep;
+    st.sweep_vacc_hour[0] = h;
+    st.sweep_vacc_thr[0] = vacc_thr;
+    st.commit_hour[1] = next;
+  }
   *S = st;
 }
 
-// The sweep decided at the previous hour (allocation_map.rs:349-387): vaccination of
-// susceptible agents (a Bernoulli keyed by the vaccination hour), lockdown isolation of
-// everyone but essential workers, unlock. Applying it twice gives the same word.
-__device__ __forceinline__ uint32_t apply_sweep(uint32_t w, uint32_t sweep, uint32_t local, const SimState& S,
-                                                const SimDesc& D) {
+// A decided sweep (allocation_map.rs:349-387): vaccination of susceptible agents (a
+// Bernoulli keyed by the vaccination hour), lockdown isolation of everyone but essential
+// workers, unlock. Applying it twice gives the same word.
+struct Sweep {
+  uint32_t mask, vacc_hour;
+  unsigned long long vacc_thr;
+};
+// The sweep in the slot of parity p (filled by the epilogue of an hour of that parity).
+__device__ __forceinline__ Sweep sweep_slot(const SimState& S, uint32_t p) {
+  return Sweep{S.sweep[p], S.sweep_vacc_hour[p], S.sweep_vacc_thr[p]};
+}
+__device__ __forceinline__ uint32_t apply_sweep(uint32_t w, const Sweep& sw, uint32_t local, const SimDesc& D) {
+  const uint32_t sweep = sw.mask;
   if ((sweep & SWEEP_VACC) && fld(w, B_STATE, M_STATE) == ST_S) {
-    const u32x4 d = philox4x32_10(local, S.vacc_hour, SITE_VACCINATE, 0, D.seed_lo, D.seed_hi);
-    if (bernoulli(S.vacc_thr, join64(d.w[0], d.w[1]))) w |= F_VACC;
+    const u32x4 d = philox4x32_10(local, sw.vacc_hour, SITE_VACCINATE, 0, D.seed_lo, D.seed_hi);
+    if (bernoulli(sw.vacc_thr, join64(d.w[0], d.w[1]))) w |= F_VACC;
   }
   if ((sweep & SWEEP_LOCK) && fld(w, B_WS, M_WS) != WS_ESSENTIAL) w |= F_ISO;
   if (sweep & SWEEP_UNLOCK) w &= ~F_ISO;
@@ -592,16 +620,17 @@ __device__ __forceinline__ uint32_t apply_sweep(uint32_t w, uint32_t sweep, uint
 }
 
 // Block-wide hour epilogue of one simulation: reduce the hour's per-block partial counts
-// (hours of day 2..6 repeat the previous row: nothing changed), then hour_epilogue on
-// thread 0, whose other inputs are loaded first so they arrive with the partials.
+// (hours of day 1..6 repeat the last counts: nobody's state changes), then hour_epilogue
+// on thread 0, whose other inputs are loaded first so they arrive with the partials.
 __device__ void run_epilogue(const DevView& v, uint32_t sim, const SimDesc& D, SimState* S, uint32_t h) {
   __shared__ uint32_t s_sum[6];
   EpilogueIn in;
-  if (h % 24u >= 2 && h % 24u <= 6) {
+  if (h % 24u >= 1 && h % 24u <= 6) {
     if (threadIdx.x == 0) {
       load_epilogue_in(v, sim, in);
-      const Counts7 prev = v.rows[D.row_off + (h - 2)];
-      const uint32_t c[6] = {prev.s, prev.e, prev.i, prev.h, prev.r, prev.d};
+      uint32_t c[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) c[k] = in.st.last_counts[k];
       hour_epilogue(v, in, S, h, c);
     }
     return;
@@ -625,48 +654,39 @@ __device__ void run_epilogue(const DevView& v, uint32_t sim, const SimDesc& D, S
   if (threadIdx.x == 0) hour_epilogue(v, in, S, h, s_sum);
 }
 
-__global__ void k_step(DevView v) {
+// At most 72 SGPRs. Measured at 1024-thread blocks: at 76 SGPRs (the compiler's own
+// choice, once the Philox round keys were kept live across the kernel) every k_step hour
+// ran 10-25% slower than at 66, although the compiler reports 8 waves per SIMD for both;
+// capped at 72 (a few SGPRs spilled to VGPR lanes) it runs as fast as at 66. Keeping
+// resident a whole second 16-wave block per CU depends on it.
+__global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   const BlockRef bm = v.agent_blocks[blockIdx.x];
   const uint32_t sim = first_lane(bm.sim);
   const SimDesc& D = v.desc[sim];
   SimState* S = v.state + sim;
   if (!S->active) return;  // uniform over the block
-  const uint32_t h = S->hour + 1;
+  const uint32_t h = S->hour + 1;  // an hour of day 0 or 7..23
   const uint32_t hd = h % 24u;
-  const uint32_t sweep = S->sweep;  // decided at hour h - 1
+  const Sweep sweep = sweep_slot(*S, (h - 1u) & 1u);  // decided at hour h - 1
   const uint32_t local = first_lane(bm.first) + threadIdx.x;
   const bool valid = local < D.n;
-  // Hours of day 2..6 change nothing: hour 1 already set every non-staff current area to
-  // home, and no movement or disease update runs while asleep (citizen/mod.rs:244-248).
-  // k_epilogue repeats the previous hour's counts for them; only a pending sweep is applied.
-  if (hd >= 2 && hd <= 6) {
-    ClaimShare share{nullptr, 0, 0};
-    if (hd == 2) share = claim_share(v, D, blockIdx.x - D.block_begin);
-    if (sweep && valid) {
-      const uint64_t gi = D.agent_off + local;
-      const uint32_t w = v.st[gi];
-      const uint32_t nw = apply_sweep(w, sweep, local, *S, D);
-      if (nw != w) v.st[gi] = nw;
-    }
-    reset_claims(share);
-    return;
-  }
   const uint8_t* occ = v.occ + D.cell_off;
 
   uint32_t cat = 6;
   if (valid) {
     const uint64_t gi = D.agent_off + local;
-    // st and pos are loaded together, then the work that needs neither, then one wait
-    // (wsa and tm are read by few agents and loaded where they are needed).
+    // st and pos are loaded together, then the work that needs neither (the buildings and
+    // the hour's movement draw), then one wait (wsa and tm are read by few agents and
+    // loaded where they are needed).
     uint32_t w_stored = v.st[gi];
     uint32_t p = v.pos[gi];
     Agent a(D, *S, occ);
     a.id = local;
     a.h = h;
     a.locate_buildings();
-    if (hd == 0 || hd >= 7) a.mv = a.dr(SITE_MOVE, 0);
-    loaded_together(w_stored, p);
-    a.w = sweep ? apply_sweep(w_stored, sweep, local, *S, D) : w_stored;
+    a.mv = a.dr(SITE_MOVE, 0);
+    loaded_together(w_stored, p, a.mv);
+    a.w = sweep.mask ? apply_sweep(w_stored, sweep, local, D) : w_stored;
     a.x = static_cast<int32_t>(p & 0xFFFFu);
     a.y = static_cast<int32_t>(p >> 16);
     a.tx = a.x;
@@ -691,8 +711,6 @@ __global__ void k_step(DevView v) {
         a.w = set_fld(a.w, B_DAY, M_DAY, day < kMaxDay ? day + 1 : day);
       }
       a.hospitalize();
-    } else if (hd <= 6) {
-      if (!staff) a.set_area(AREA_HOME);
     } else if (hd == 23) {
       const uint32_t s0 = a.state();
       a.deceased();
@@ -721,7 +739,7 @@ __global__ void k_step(DevView v) {
   }
 
   // S/E/I/H/R/D via wavefront ballots into LDS, then one plain store per block and
-  // category; k_epilogue reduces the block partials.
+  // category; the epilogue reduces the block partials.
   __shared__ uint32_t s_counts[6];
   if (threadIdx.x < 6) s_counts[threadIdx.x] = 0;
   __syncthreads();
@@ -746,10 +764,29 @@ __device__ __forceinline__ void epilogue_block(const DevView& v, uint32_t sim) {
   run_epilogue(v, sim, v.desc[sim], S, S->hour + 1);
 }
 
-__global__ void k_epilogue(DevView v) { epilogue_block(v, blockIdx.x); }
+// Agent pass of an hour of day 1..6 (block b of the simulation's agent blocks): nobody
+// moves and no disease update runs while asleep (citizen/mod.rs:244-248), so only the
+// sweep decided at hour h - 1 is applied, hour of day 1 sets every non-staff current area
+// to home, and hour of day 2 resets the claim array before the day's first claim hour.
+__device__ __forceinline__ void rest_pass(const DevView& v, const SimDesc& D, const SimState& S, uint32_t h,
+                                          uint32_t local, uint32_t b) {
+  const uint32_t hd = h % 24u;
+  const Sweep sweep = sweep_slot(S, (h - 1u) & 1u);
+  ClaimShare share{nullptr, 0, 0};
+  if (hd == 2) share = claim_share(v, D, b);
+  if (local < D.n && (sweep.mask || hd == 1)) {
+    const uint64_t gi = D.agent_off + local;
+    const uint32_t w = v.st[gi];
+    uint32_t nw = sweep.mask ? apply_sweep(w, sweep, local, D) : w;
+    if (hd == 1 && fld(nw, B_WS, M_WS) != WS_STAFF) nw = set_fld(nw, B_AREA, M_AREA, AREA_HOME);
+    if (nw != w) v.st[gi] = nw;
+  }
+  reset_claims(share);
+}
 
 // `parity` is the hour's parity (an hour of day 24k + j has the parity of j). Blocks
-// 0 .. n_epi-1 run the epilogue of simulation blockIdx.x; the rest commit agent blocks.
+// 0 .. n_epi-1 run the epilogue of simulation blockIdx.x; the rest run the agent pass:
+// the commit at hours of day 0 and 7..23, rest_pass at 1..6.
 __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
   if (blockIdx.x < n_epi) {  // uniform
     epilogue_block(v, blockIdx.x);
@@ -762,6 +799,10 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
   const uint32_t h = S->commit_hour[parity];
   if (h == 0) return;  // uniform: the simulation did not step this hour
   const uint32_t local = first_lane(bm.first) + threadIdx.x;
+  if (h % 24u >= 1 && h % 24u <= 6) {  // uniform
+    rest_pass(v, D, *S, h, local, blockIdx.x - n_epi - D.block_begin);
+    return;
+  }
   if (local < D.n) {
     const uint64_t gi = D.agent_off + local;
     // st, pos and tgt are loaded together so a mover pays two dependent round trips
@@ -791,18 +832,18 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
 }
 
 // Re-applies each simulation's last decided sweep (idempotent) so that agent state read
-// by the host includes it; the next k_step applies it again to the same words.
+// by the host includes it; the next hour's agent pass applies it again to the same words.
 __global__ void k_apply_sweep(DevView v) {
   const BlockRef bm = v.agent_blocks[blockIdx.x];
   const uint32_t sim = first_lane(bm.sim);
   const SimDesc& D = v.desc[sim];
   const SimState* S = v.state + sim;
-  const uint32_t sweep = S->sweep;
+  const Sweep sweep = sweep_slot(*S, S->hour & 1u);
   const uint32_t local = first_lane(bm.first) + threadIdx.x;
-  if (sweep == 0 || local >= D.n) return;
+  if (sweep.mask == 0 || local >= D.n) return;
   const uint64_t gi = D.agent_off + local;
   const uint32_t w = v.st[gi];
-  const uint32_t nw = apply_sweep(w, sweep, local, *S, D);
+  const uint32_t nw = apply_sweep(w, sweep, local, D);
   if (nw != w) v.st[gi] = nw;
 }
 
@@ -853,12 +894,6 @@ hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStrea
 hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream) {
   if (s.agent_blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_step, dim3(s.agent_blocks), dim3(s.block), 0, stream, v);
-  return hipGetLastError();
-}
-
-hipError_t launch_epilogue(const DevView& v, const LaunchShape& s, hipStream_t stream) {
-  if (s.n_sims == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_epilogue, dim3(s.n_sims), dim3(1024), 0, stream, v);
   return hipGetLastError();
 }
 

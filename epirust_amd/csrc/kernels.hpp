@@ -37,16 +37,16 @@ struct LaunchShape {
   uint32_t n_sims = 0;
 };
 
-// One hour: [hour of day 0] scan, step, then either the commit with the epilogue in its
-// first blocks (hours of day 0 and 7..23) or the epilogue alone (hours of day 1..6).
+// One hour: [hour of day 0] scan, [hours of day 0 and 7..23] step, then the commit launch:
+// the epilogue in its first blocks and the hour's agent pass in the rest (the commit at
+// hours of day 0 and 7..23, the pending sweep / area reset / claim reset at 1..6).
 // `hour` only needs the right parity, so a captured day of hours 1..24 replays for any day.
 hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStream_t stream);
 hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream);
-hipError_t launch_epilogue(const DevView& v, const LaunchShape& s, hipStream_t stream);
 hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, hipStream_t stream);
 // Re-applies the last decided sweep of every simulation (idempotent), before agent read-back.
 hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_t stream);
-inline bool hour_has_commit(uint32_t hour) { return hour % 24u == 0 || hour % 24u >= 7; }
+inline bool hour_has_step(uint32_t hour) { return hour % 24u == 0 || hour % 24u >= 7; }
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t stream);
 
 }  // namespace epi

@@ -60,7 +60,7 @@ enum : uint32_t {
   SITE_INIT_HOUSE = 19 // counter id = house index; w0,w1 x permutation, w2,w3 y permutation
 };
 
-// ---- sweeps decided after an hour's counts, applied by the commit kernel
+// ---- sweeps decided after an hour's counts, applied by the next hour's agent pass
 enum : uint32_t { SWEEP_VACC = 1, SWEEP_LOCK = 2, SWEEP_UNLOCK = 4 };
 // ---- intervention event kinds (intervention_reporter.rs names)
 enum : uint32_t { EV_VACCINATION = 0, EV_LOCKED = 1, EV_UNLOCKED = 2, EV_HOSPITAL = 3 };
@@ -114,16 +114,19 @@ struct SimDesc {
 struct SimState {
   uint32_t hour;            // last hour stepped
   uint32_t active;          // 0 once stopped (E=I=H=0) or out of hours
-  uint32_t commit_hour[2];  // [hour & 1]: the hour the commit of that parity resolves, 0 = none
-  uint32_t sweep;           // SWEEP_* decided at `hour`, applied by the next step (idempotent)
+  uint32_t commit_hour[2];  // [hour & 1]: the hour the agent pass of that parity runs, 0 = none
+  // [hour & 1]: the sweep decided at that hour (SWEEP_*), applied by the next hour's agent
+  // pass (idempotent), with the hour and threshold of its vaccination draw
+  uint32_t sweep[2];
+  uint32_t sweep_vacc_hour[2];
+  uint64_t sweep_vacc_thr[2];
   uint32_t first_vacant;    // row-major index in the hospital, or ~0u
   uint32_t hosp_version;    // 0 after resize_hospital, 1 after increase_hospital_size
   uint32_t locked, zero_infection_hour;
   uint32_t hosp_new_infections, hosp_applied;
-  uint32_t vacc_hour;
   uint32_t vacc_next;       // first schedule entry not yet passed
   uint32_t n_events;
-  uint64_t vacc_thr;
+  uint32_t last_counts[6];  // S E I H R D of the last row (the population's before hour 1)
   uint32_t rows_written;
   uint32_t error;
 };

@@ -18,7 +18,7 @@ import json
 import re
 import sys
 
-HOUR_KERNELS = ("k_hospital_scan", "k_step", "k_epilogue", "k_commit")
+HOUR_KERNELS = ("k_hospital_scan", "k_step", "k_commit")
 
 
 def short(name):
@@ -32,8 +32,9 @@ def short(name):
 
 def by_hour_of_day(trace_csv, hours):
     """Mean duration per kernel for each hour of day over the first `hours` hours of the
-    trace (hours are counted by k_step dispatches; the hour-0 hospital scan runs just
-    before the step of hour 24k), with the wall time from one step to the next."""
+    trace (an hour is [k_hospital_scan] [k_step] k_commit: hours are counted by k_commit
+    dispatches, which every hour has), with the wall time from one hour's first kernel to
+    the next hour's."""
     ks = []
     for r in csv.DictReader(open(trace_csv)):
         k = short(r["Kernel_Name"])
@@ -42,22 +43,19 @@ def by_hour_of_day(trace_csv, hours):
     ks.sort()
     dur = collections.defaultdict(lambda: collections.defaultdict(list))
     start = {}
-    hour = 0
+    hour = 1
     for s, e, k in ks:
-        if k == "k_hospital_scan":
-            dur[(hour + 1) % 24][k].append((e - s) / 1e3)
-            continue
-        if k == "k_step":
-            hour += 1
-            if hour > hours:
-                break
-            start[hour] = s
+        if hour > hours:
+            break
+        start.setdefault(hour, s)
         dur[hour % 24][k].append((e - s) / 1e3)
+        if k == "k_commit":
+            hour += 1
     wall = collections.defaultdict(list)
     for h in range(1, hours):
         if h + 1 in start:
             wall[h % 24].append((start[h + 1] - start[h]) / 1e3)
-    out = ["| hour of day | " + " | ".join(HOUR_KERNELS) + " | kernels µs | step-to-step µs |",
+    out = ["| hour of day | " + " | ".join(HOUR_KERNELS) + " | kernels µs | hour-to-hour µs |",
            "|---" * (len(HOUR_KERNELS) + 3) + "|"]
     tot = 0.0
     for hd in range(24):
@@ -66,7 +64,7 @@ def by_hour_of_day(trace_csv, hours):
         tot += sum(means)
         out.append("| %d | " % hd + " | ".join("%.1f" % m for m in means) + " | %.1f | %.1f |" % (sum(means), w))
     out.append("")
-    out.append("Mean kernel time per simulated hour: %.1f µs. Step-to-step times include the profiler's "
+    out.append("Mean kernel time per simulated hour: %.1f µs. Hour-to-hour times include the profiler's "
                "own per-dispatch overhead; the untraced hour is shorter." % (tot / 24))
     return "\n".join(out) + "\n"
 

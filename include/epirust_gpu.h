@@ -132,8 +132,9 @@ int epi_geometry(epi_sim* sim, uint32_t index, int32_t rects[20], uint32_t* hosp
 
 /* Profiling: runs n_hours with a HIP event pair around every kernel on the handle's stream
  * and returns the mean duration in ms of each of an hour's kernels (hospital scan; step;
- * commit with its folded epilogue, or the epilogue alone at hours of day 1..6; the hospital
- * scan mean is over the hours that ran it) and how many hours ran. */
+ * the commit launch with its folded epilogue), each mean over the hours that launched it
+ * (scan: hour of day 0; step: hours of day 0 and 7..23; commit: every hour), the number of
+ * launches of each, and how many hours ran. */
 int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t launches[3], uint32_t* hours_run);
 
 /* Launch shape: threads per agent block (64..1024, multiple of 64). Takes effect on the
