@@ -296,7 +296,7 @@ def main():
                                "deterministic mode" % (n, g, g, run_hours, cfg.get("seed", 0)),
                    "agents": n, "grid_size": g, "hours": run_hours,
                    "threads_per_block": args.block or tuned_block_of(cfg) or 256,
-                   "kernel_ms": {"hospital_scan": scan_ms, "step": step_ms, "commit_or_epilogue": commit_ms}},
+                   "kernel_ms": {"hospital_scan": scan_ms, "step": step_ms, "commit": commit_ms}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "one simulated hour: k_commit with its folded epilogue every hour, k_step at hours of day 0 and 7..23, k_hospital_scan once a day",
