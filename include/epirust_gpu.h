@@ -67,7 +67,9 @@ typedef struct epi_sim epi_sim;
 /* Replaces Epidemiology::new (engine/src/epidemiology_simulation.rs:75-135), called from
  * EngineApp::start_standalone (engine/src/engine_app.rs:89-125) with the config read by
  * Config::read (common/src/config/mod.rs:119-123). config_json is the reference's
- * standalone config schema; `seed` keys the Philox streams. */
+ * standalone config schema; `seed` keys the Philox streams. A simulation holds at most
+ * 2^27 - 1 agents (agent ids share a 32-bit claim word with the claim hour): larger
+ * populations return EPI_ERR_CONFIG. */
 int epi_create(const char* config_json, uint64_t seed, uint32_t flags, epi_sim** out);
 
 /* Batched replicates: n independent simulations sharing every launch. Same role as the
