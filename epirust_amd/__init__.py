@@ -82,6 +82,7 @@ SYMBOLS = {
     "epi_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_double), _P(ctypes.c_uint32),
                                    _P(ctypes.c_uint32)]),
     "epi_set_block_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "epi_last_run_ms": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), _P(ctypes.c_uint32)]),
     "epi_citizen_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
                                           _P(ctypes.c_uint32), _P(ctypes.c_uint32)]),
     "epi_stream_citizen_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
@@ -210,6 +211,13 @@ class _Handle:
 
     def set_block_size(self, threads):
         _check(lib().epi_set_block_size(self._h, threads))
+
+    def last_run_ms(self):
+        """(device ms, hours) of the last step call's hour loop, from HIP events on the
+        handle's stream."""
+        ms, hours = ctypes.c_double(), ctypes.c_uint32()
+        _check(lib().epi_last_run_ms(self._h, ctypes.byref(ms), ctypes.byref(hours)))
+        return ms.value, hours.value
 
     def citizen_states(self, index=0):
         """(hour, records) after the last stepped hour: a structured array of

@@ -140,6 +140,9 @@ struct epi_sim {
   hipStream_t stream = nullptr;
   hipGraphExec_t day_graph = nullptr;
   hipEvent_t poll_event[2] = {nullptr, nullptr};
+  hipEvent_t loop_event[2] = {nullptr, nullptr};  // around the hour loop of one step call
+  double last_loop_ms = 0.0;                      // device time of the last step call's hours
+  uint32_t last_loop_hours = 0;
   uint32_t* pinned_active = nullptr;  // 2 x n_sims
   uint32_t host_hour = 0;             // hours enqueued so far (every active simulation is at this hour)
   bool all_stopped = false;           // every simulation reported inactive
@@ -150,6 +153,8 @@ struct epi_sim {
   ~epi_sim() {
     if (day_graph) (void)hipGraphExecDestroy(day_graph);
     for (auto& e : poll_event)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : loop_event)
       if (e) (void)hipEventDestroy(e);
     if (pinned_active) (void)hipHostFree(pinned_active);
     if (stream) (void)hipStreamDestroy(stream);
@@ -321,10 +326,14 @@ int advance_streaming(epi_sim* s, uint32_t target) {
 // Enqueue hours until `target` (or until every simulation has stopped). Whole days are
 // replayed from the captured graph; stop flags are polled one day behind the GPU.
 int advance(epi_sim* s, uint32_t target) {
+  s->last_loop_ms = 0.0;
+  s->last_loop_hours = 0;
   if (s->any_empty && target > s->host_hour) return fail(EPI_ERR_PANIC, "No citizens!");
   if (s->all_stopped) return EPI_OK;
   if (s->sink) return advance_streaming(s, target);
   const bool graph_ok = !(s->flags & EPI_FLAG_NO_GRAPH);
+  const uint32_t first_hour = s->host_hour;
+  HIP_TRY(hipEventRecord(s->loop_event[0], s->stream));
   uint32_t day_index = 0;
   bool pending[2] = {false, false};
   while (s->host_hour < target) {
@@ -356,7 +365,12 @@ int advance(epi_sim* s, uint32_t target) {
     pending[slot] = true;
     ++day_index;
   }
+  HIP_TRY(hipEventRecord(s->loop_event[1], s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, s->loop_event[0], s->loop_event[1]));
+  s->last_loop_ms = ms;
+  s->last_loop_hours = s->host_hour - first_hour;
   return EPI_OK;
 }
 
@@ -609,6 +623,8 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_active), 2 * s->n_sims * sizeof(uint32_t),
                         hipHostMallocDefault));
   HIP_TRY(hipEventCreateWithFlags(&s->poll_event[0], hipEventDisableTiming));
+  HIP_TRY(hipEventCreate(&s->loop_event[0]));
+  HIP_TRY(hipEventCreate(&s->loop_event[1]));
   HIP_TRY(hipEventCreateWithFlags(&s->poll_event[1], hipEventDisableTiming));
   HIP_TRY(hipStreamSynchronize(s->stream));
   *out = s.release();
@@ -1011,6 +1027,13 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
   rc = fetch_states(sim);
   if (rc != EPI_OK) return rc;
   for (uint32_t i = 0; i < sim->n_sims; ++i) sim->rows_returned[i] = sim->state_h[i].rows_written;
+  return EPI_OK;
+}
+
+int epi_last_run_ms(epi_sim* sim, double* device_ms, uint32_t* hours) {
+  if (!sim || !device_ms || !hours) return fail(EPI_ERR_ARG, "null argument");
+  *device_ms = sim->last_loop_ms;
+  *hours = sim->last_loop_hours;
   return EPI_OK;
 }
 

@@ -139,6 +139,12 @@ int epi_geometry(epi_sim* sim, uint32_t index, int32_t rects[20], uint32_t* hosp
  * launches of each, and how many hours ran. */
 int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t launches[3], uint32_t* hours_run);
 
+/* Device time of the hour loop of the last epi_step / epi_batch_step / epi_run call: HIP
+ * events recorded on the handle's stream before its first hour's launches and after its
+ * last hour's, and the number of hours enqueued between them (0 and 0 when no hour ran or
+ * the citizen-state stream was open). Measurement only; no reference counterpart. */
+int epi_last_run_ms(epi_sim* sim, double* device_ms, uint32_t* hours);
+
 /* Launch shape: threads per agent block (64..1024, multiple of 64). Takes effect on the
  * next step; discards the captured day graph. */
 int epi_set_block_size(epi_sim* sim, uint32_t threads);
