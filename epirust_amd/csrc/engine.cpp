@@ -134,7 +134,10 @@ struct epi_sim {
   std::vector<uint32_t> vacc_hour;
   std::vector<unsigned long long> vacc_thr;
   DeviceBuffers mem;
-  DevView v{};
+  DevView v{};         // agent arrays in the home layout
+  DevView v_office{};  // the same view over the office-layout copies (hours of day 8..15)
+  TileRef* d_tiles = nullptr;  // layout transpose tiles of every simulation
+  uint32_t n_tiles = 0;
   BlockRef* d_agent_blocks = nullptr;
   SimDesc* d_desc = nullptr;
   hipStream_t stream = nullptr;
@@ -204,7 +207,7 @@ int build_launch_tables(epi_sim* s) {
   const uint32_t B = s->shape.block;
   s->agent_blocks.clear();
   for (uint32_t i = 0; i < s->n_sims; ++i) {
-    const uint32_t n = s->desc[i].n;
+    const uint32_t n = s->desc[i].n_slots;
     const uint32_t nb = (n + B - 1) / B;
     s->desc[i].n_blocks = nb;
     s->desc[i].block_begin = static_cast<uint32_t>(s->agent_blocks.size());
@@ -225,10 +228,34 @@ int upload_tables(epi_sim* s) {
   return EPI_OK;
 }
 
+// Copies the agent arrays into the other layout.
+int relayout(epi_sim* s, bool to_office) {
+  HIP_TRY(launch_relayout(s->v, s->v_office, s->d_tiles, s->n_tiles, to_office, s->stream));
+  return EPI_OK;
+}
+
+// Before the host reads agents: the home layout holds the current state (it is copied back
+// from the office layout after an hour of day 8..15) and includes the last decided sweep.
+int settle(epi_sim* s) {
+  if (s->host_hour > 0 && office_hour(s->host_hour)) {
+    const int rc = relayout(s, false);
+    if (rc != EPI_OK) return rc;
+  }
+  HIP_TRY(launch_apply_sweep(s->v, s->shape, s->stream));
+  return EPI_OK;
+}
+
 int enqueue_hour(epi_sim* s, uint32_t next_hour) {
-  if (next_hour % 24u == 0) HIP_TRY(launch_hospital_scan(s->v, s->shape, s->stream));
-  if (hour_has_step(next_hour)) HIP_TRY(launch_step(s->v, s->shape, s->stream));
-  HIP_TRY(launch_commit(s->v, s->shape, next_hour, s->stream));
+  const uint32_t hd = next_hour % 24u;
+  if (hd == 0) HIP_TRY(launch_hospital_scan(s->v, s->shape, s->stream));
+  // the office layout holds the agent arrays from hour of day 8's step to 15's commit
+  if (hd == 8 || hd == 16) {
+    const int rc = relayout(s, hd == 8);
+    if (rc != EPI_OK) return rc;
+  }
+  const DevView& v = office_hour(next_hour) ? s->v_office : s->v;
+  if (hour_has_step(next_hour)) HIP_TRY(launch_step(v, s->shape, s->stream));
+  HIP_TRY(launch_commit(v, s->shape, next_hour, s->stream));
   return EPI_OK;
 }
 
@@ -290,6 +317,10 @@ int advance_streaming(epi_sim* s, uint32_t target) {
     const uint32_t h = s->host_hour + 1;
     rc = enqueue_hour(s, h);
     if (rc != EPI_OK) return rc;
+    if (office_hour(h)) {  // the records are formatted from the home layout
+      rc = relayout(s, false);
+      if (rc != EPI_OK) return rc;
+    }
     a.hour = h;
     HIP_TRY(launch_citizen_states(a, s->stream));
     HIP_TRY(hipMemcpyAsync(k.h_total + slot, k.d_total, sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -380,7 +411,8 @@ int step_impl(epi_sim* s, uint32_t n_hours, epi_counts* rows, uint32_t row_strid
   const uint32_t target = static_cast<uint32_t>(std::min<uint64_t>(want, s->max_hour));
   int rc = advance(s, target);
   if (rc != EPI_OK) return rc;
-  HIP_TRY(launch_apply_sweep(s->v, s->shape, s->stream));  // agent state includes the last hour's sweep
+  rc = settle(s);  // agent state in the home layout, with the last hour's sweep
+  if (rc != EPI_OK) return rc;
   rc = fetch_states(s);
   if (rc != EPI_OK) return rc;
   s->all_stopped = true;
@@ -487,6 +519,10 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     D.fd_house_nx = make_fastdiv(static_cast<uint32_t>(D.house_nx));
     D.fd_offices = make_fastdiv(D.n_offices);
     D.fd_office_nx = make_fastdiv(static_cast<uint32_t>(D.office_nx));
+    // agent slots: n_offices x per_office in both layouts (holes past n)
+    D.per_office = (D.n + D.n_offices - 1u) / D.n_offices;
+    D.n_slots = D.n_offices * D.per_office;
+    D.fd_per_office = make_fastdiv(std::max(D.per_office, 1u));
     D.regular_start = c.regular_transmission_start_day;
     D.high_start = c.high_transmission_start_day;
     D.last_day = c.last_day;
@@ -519,7 +555,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
       if (!rect_inside(*r, G.stride, G.height))
         return fail(EPI_ERR_CONFIG, "grid_size too small for the engine's area layout");
     if (c.number_of_agents == 0 && c.hours > 1) s->any_empty = true;
-    agent_off += D.n;
+    agent_off += D.n_slots;
     cell_off += padded_cells(G.stride, G.height);
     row_off += c.hours > 1 ? c.hours - 1 : 0;
     event_off += D.event_cap;
@@ -551,6 +587,9 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   HIP_TRY(M.alloc(&v.tm, s->n_agents));
   HIP_TRY(M.alloc(&v.wsa, s->n_agents));
   HIP_TRY(M.alloc(&v.tgt, s->n_agents));
+  DevView& vo = s->v_office;  // completed from v below
+  HIP_TRY(M.alloc(&vo.pos, s->n_agents));
+  HIP_TRY(M.alloc(&vo.st, s->n_agents));
   HIP_TRY(M.alloc(&v.occ, s->n_cells));
   HIP_TRY(M.alloc(&v.claim, s->n_cells));
   HIP_TRY(M.alloc(&s->d_desc, s->n_sims));
@@ -558,7 +597,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   HIP_TRY(M.alloc(&v.state, s->n_sims));
   // table sized for the smallest legal block (64 threads) so block-size changes never reallocate
   uint64_t max_blocks = 0;
-  for (uint32_t i = 0; i < s->n_sims; ++i) max_blocks += (s->desc[i].n + 63) / 64;
+  for (uint32_t i = 0; i < s->n_sims; ++i) max_blocks += (s->desc[i].n_slots + 63) / 64;
   HIP_TRY(M.alloc(&s->d_agent_blocks, max_blocks));
   HIP_TRY(M.alloc(&v.block_counts, 6 * max_blocks));
   v.agent_blocks = s->d_agent_blocks;
@@ -618,6 +657,22 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
                       hipMemcpyHostToDevice));
   }
   HIP_TRY(launch_fill_u32(v.claim, s->n_cells, ~0u, s->stream));
+  {  // the office-layout view shares everything with v but pos and st
+    DevView o = v;
+    o.pos = vo.pos;
+    o.st = vo.st;
+    o.office = 1;
+    v.office = 0;
+    vo = o;
+  }
+  std::vector<TileRef> tiles;
+  for (uint32_t i = 0; i < s->n_sims; ++i)
+    for (uint32_t k0 = 0; k0 < s->desc[i].per_office; k0 += 32)
+      for (uint32_t j0 = 0; j0 < s->desc[i].n_offices; j0 += 32) tiles.push_back(TileRef{i, k0, j0});
+  s->n_tiles = static_cast<uint32_t>(tiles.size());
+  HIP_TRY(M.alloc(&s->d_tiles, tiles.size()));
+  if (!tiles.empty())
+    HIP_TRY(hipMemcpy(s->d_tiles, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   rc = upload_tables(s.get());
   if (rc != EPI_OK) return rc;
   HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_active), 2 * s->n_sims * sizeof(uint32_t),
@@ -995,9 +1050,12 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
       }
       (void)hipEventRecord(e[1], sim->stream);
       had_step[k] = hour_has_step(h) ? 1 : 0;
-      if (had_step[k] && launch_step(sim->v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      // the layout transposes of hours of day 8 and 16 are timed with the step
+      if ((h % 24u == 8 || h % 24u == 16) && rc == EPI_OK) rc = relayout(sim, h % 24u == 8);
+      const DevView& v = office_hour(h) ? sim->v_office : sim->v;
+      if (had_step[k] && launch_step(v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       (void)hipEventRecord(e[2], sim->stream);
-      if (launch_commit(sim->v, sim->shape, h, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      if (launch_commit(v, sim->shape, h, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       (void)hipEventRecord(e[3], sim->stream);
       ++sim->host_hour;
     }
@@ -1022,7 +1080,8 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
     launches[k] = cnt[k];
   }
   if (hours_run) *hours_run = ran;
-  HIP_TRY(launch_apply_sweep(sim->v, sim->shape, sim->stream));
+  rc = settle(sim);
+  if (rc != EPI_OK) return rc;
   // keep row accounting consistent for later epi_step calls
   rc = fetch_states(sim);
   if (rc != EPI_OK) return rc;

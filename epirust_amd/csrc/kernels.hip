@@ -34,11 +34,23 @@ namespace epi {
 namespace {
 
 constexpr int kWave = 64;
-// Agents of one simulation that the chip holds at once (256 CUs x 2048 threads). Blocks
-// are dispatched in agent order, so agents below this id run in the first residency round,
-// concurrently with every lower id: a claim pre-check there finds no earlier claims and
-// only adds a load.
+// Agent slots of one simulation that the chip holds at once (256 CUs x 2048 threads).
+// Blocks are dispatched in slot order, so slots below this run in the first residency
+// round, concurrently with every slot before them: a claim pre-check there finds no
+// earlier claims and only adds a load. (In both layouts the lower ids among an office's or
+// a house's agents, the ones a pre-check looks for, sit in earlier slots.)
 constexpr uint32_t kResidentAgents = 256u * 2048u;
+// Office-hour block claim filter: 100 cells per 10 x 10 office, at most kFilterCells words
+// of LDS (a block of up to 1024 slots covers at most 20 offices when per_office >= 52).
+constexpr uint32_t kOfficeCells = 100u, kFilterCells = 2048u, kNoCell = ~0u;
+
+// Agent id of a slot: the identity in the home layout; slot = office * per_office + k
+// holds agent k * n_offices + office in the office layout.
+__device__ __forceinline__ uint32_t agent_of_slot(const DevView& v, const SimDesc& D, uint32_t slot) {
+  if (!v.office) return slot;
+  const uint32_t office = fast_div(slot, D.fd_per_office);
+  return (slot - office * D.per_office) * D.n_offices + office;
+}
 
 __device__ __forceinline__ uint32_t first_lane(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -150,6 +162,7 @@ struct Agent {
   // claim. Contended cells are mostly claimed first by lower ids in earlier blocks, so for
   // agents past the first residency round this removes most same-address atomics.
   uint32_t seen_claim = ~0u;
+  bool precheck = false;  // this agent's block runs after the first residency round
   int32_t x, y;                     // current cell
   int32_t tx, ty;                   // chosen cell
   int32_t hx = 0, hy = 0;           // home house origin (house id % H)
@@ -310,7 +323,7 @@ struct Agent {
     const int k = __builtin_ctz(cand);
     tx = lx + moore_dx(k);
     ty = ly + moore_dy(k);
-    if (id >= kResidentAgents) seen_claim = claims[cell_index(D, tx, ty)];  // overlaps the disease update
+    if (precheck) seen_claim = claims[cell_index(D, tx, ty)];  // overlaps the disease update
   }
 
   // default_disease_handler.rs:64-86 (OR over fixed per-neighbour draws)
@@ -668,13 +681,24 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   const uint32_t h = S->hour + 1;  // an hour of day 0 or 7..23
   const uint32_t hd = h % 24u;
   const Sweep sweep = sweep_slot(*S, (h - 1u) & 1u);  // decided at hour h - 1
-  const uint32_t local = first_lane(bm.first) + threadIdx.x;
-  const bool valid = local < D.n;
+  const uint32_t slot = first_lane(bm.first) + threadIdx.x;
+  const uint32_t local = agent_of_slot(v, D, slot);  // the agent id
+  const bool valid = slot < D.n_slots && local < D.n;
   const uint8_t* occ = v.occ + D.cell_off;
 
   uint32_t cat = 6;
+  // This lane's claim and state word, carried past the block's claim filter to the stores.
+  bool moves = false;
+  uint32_t key = 0, w_out = 0, w_in = 0, tgt_word = 0, office_cell = kNoCell;
+  const uint64_t gi = D.agent_off + slot;
+  uint64_t ci = 0;
+  // Office hours: the offices this block's slots belong to (j_lo .. j_lo + n_off - 1), each
+  // with 100 cells in the block's claim filter; computed before any store.
+  const uint32_t first = first_lane(bm.first);
+  const uint32_t j_lo = v.office ? fast_div(first, D.fd_per_office) : 0u;
+  const uint32_t j_hi = v.office ? min(D.n_offices - 1u, fast_div(first + blockDim.x - 1u, D.fd_per_office)) : 0u;
+  const bool filter = v.office && (j_hi - j_lo + 1u) * kOfficeCells <= kFilterCells;
   if (valid) {
-    const uint64_t gi = D.agent_off + local;
     // st and pos are loaded together, then the work that needs neither (the buildings and
     // the hour's movement draw), then one wait (wsa and tm are read by few agents and
     // loaded where they are needed).
@@ -692,11 +716,13 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     a.tx = a.x;
     a.ty = a.y;
     a.tm = 0;
-    a.tm_arr = v.tm + gi;
+    const uint64_t gid = D.agent_off + local;  // tm and wsa stay in the home layout (by id)
+    a.tm_arr = v.tm + gid;
     a.claims = v.claim + D.cell_off;
+    a.precheck = slot >= kResidentAgents;
     const uint32_t w0 = w_stored;
     const bool staff = a.ws() == WS_STAFF;
-    a.wsa = staff ? v.wsa[gi] : 0u;
+    a.wsa = staff ? v.wsa[gid] : 0u;
     const uint32_t wsa0 = a.wsa;
     bool tm_dirty = false;
 
@@ -722,20 +748,46 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     }
     // Everything the stores need is computed first: a descriptor field read after a store
     // can no longer come from the scalar cache and costs a vector round trip.
-    const uint32_t key = claim_key(h, local);
-    const bool moves = (a.tx != a.x || a.ty != a.y) && key < a.seen_claim;
-    if (moves) a.w |= F_MOVE;
+    key = claim_key(h, local);
+    moves = (a.tx != a.x || a.ty != a.y) && key < a.seen_claim;
     if (grid_byte(D, a.w) != grid_byte(D, w0)) a.w |= F_DIRTY;
-    const uint64_t ci = D.cell_off + cell_index(D, a.tx, a.ty);
+    ci = D.cell_off + cell_index(D, a.tx, a.ty);
     const uint32_t s = a.state();
     cat = s == ST_S ? 0u : s == ST_E ? 1u : s == ST_I ? ((a.w & F_HOSP) ? 3u : 2u) : s == ST_R ? 4u : 5u;
+    w_out = a.w;
+    w_in = w0;
+    tgt_word = static_cast<uint32_t>(a.tx) | (static_cast<uint32_t>(a.ty) << 16);
+    // a worker's claim of a cell of its own office (the only agents that claim office
+    // cells at these hours) enters the filter
+    const uint32_t j = fast_mod(local, D.fd_offices);  // == slot / per_office in the office layout
+    if (filter && moves && a.tx >= a.ox && a.tx <= a.ox + 9 && a.ty >= a.oy && a.ty <= a.oy + 9 && j >= j_lo &&
+        j <= j_hi)
+      office_cell = (j - j_lo) * kOfficeCells + static_cast<uint32_t>(a.ty - a.oy) * 10u +
+                    static_cast<uint32_t>(a.tx - a.ox);
+    if (tm_dirty) v.tm[gid] = a.tm;
+    if (staff && a.wsa != wsa0) v.wsa[gid] = a.wsa;
+  }
+  // Block claim filter (office hours). The office layout puts each office's workers in one
+  // or two blocks, and they are the only claimants of its cells, so most same-cell rivals
+  // meet here: an LDS atomicMin per claimed cell leaves only the block's lowest key, and a
+  // lane that is not it has lost to a lower id and makes no global claim. At most one
+  // global atomic per cell and block remains, instead of one per rival.
+  if (filter) {  // uniform
+    __shared__ uint32_t s_cell[kFilterCells];
+    const uint32_t n_cells = (j_hi - j_lo + 1u) * kOfficeCells;
+    for (uint32_t i = threadIdx.x; i < n_cells; i += blockDim.x) s_cell[i] = ~0u;
+    __syncthreads();
+    if (office_cell != kNoCell) atomicMin(&s_cell[office_cell], key);
+    __syncthreads();
+    if (office_cell != kNoCell && s_cell[office_cell] != key) moves = false;
+  }
+  if (valid) {
     if (moves) {
+      w_out |= F_MOVE;
       atomicMin(&v.claim[ci], key);
-      v.tgt[gi] = static_cast<uint32_t>(a.tx) | (static_cast<uint32_t>(a.ty) << 16);
+      v.tgt[gi] = tgt_word;
     }
-    if (a.w != w0) v.st[gi] = a.w;
-    if (tm_dirty) v.tm[gi] = a.tm;
-    if (staff && a.wsa != wsa0) v.wsa[gi] = a.wsa;
+    if (w_out != w_in) v.st[gi] = w_out;
   }
 
   // S/E/I/H/R/D via wavefront ballots into LDS, then one plain store per block and
@@ -798,13 +850,14 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
   const SimState* S = v.state + sim;
   const uint32_t h = S->commit_hour[parity];
   if (h == 0) return;  // uniform: the simulation did not step this hour
-  const uint32_t local = first_lane(bm.first) + threadIdx.x;
-  if (h % 24u >= 1 && h % 24u <= 6) {  // uniform
-    rest_pass(v, D, *S, h, local, blockIdx.x - n_epi - D.block_begin);
+  const uint32_t slot = first_lane(bm.first) + threadIdx.x;
+  if (h % 24u >= 1 && h % 24u <= 6) {  // uniform; home layout
+    rest_pass(v, D, *S, h, slot, blockIdx.x - n_epi - D.block_begin);
     return;
   }
-  if (local < D.n) {
-    const uint64_t gi = D.agent_off + local;
+  const uint32_t local = agent_of_slot(v, D, slot);  // the agent id
+  if (slot < D.n_slots && local < D.n) {
+    const uint64_t gi = D.agent_off + slot;
     // st, pos and tgt are loaded together so a mover pays two dependent round trips
     // (these three, then its claim), not four.
     uint32_t w = v.st[gi];
@@ -877,6 +930,47 @@ __global__ void k_hospital_scan(DevView v, uint32_t cells_per_thread) {
   if ((threadIdx.x & (kWave - 1)) == 0 && best != ~0u) atomicMin(&S->first_vacant, best);
 }
 
+// Layout transpose of one tile: agents k0..k0+31 (k = id / n_offices) of offices
+// j0..j0+31 (j = id % n_offices), for st and pos (tm and wsa, which few agents read, stay
+// in the home layout and are indexed by id in both). A simulation's home slot
+// k * n_offices + j and office slot j * per_office + k hold the same agent, so the layouts
+// are the two row-major orders of a per_office x n_offices matrix. Reads and writes are
+// both coalesced (32 consecutive words per row); the turn goes through LDS, whose 33-word
+// rows keep the column reads free of bank conflicts.
+__global__ void __launch_bounds__(256) k_relayout(DevView home, DevView office, const TileRef* tiles,
+                                                  uint32_t to_office) {
+  __shared__ uint32_t tile[2][32][33];  // [array][k - k0][j - j0]
+  const TileRef tr = tiles[blockIdx.x];
+  const SimDesc& D = home.desc[first_lane(tr.sim)];
+  const uint32_t K = D.per_office, O = D.n_offices;
+  const uint64_t base = D.agent_off;
+  const uint32_t k0 = first_lane(tr.k0), j0 = first_lane(tr.j0);
+  const uint32_t lx = threadIdx.x & 31u, ly = threadIdx.x >> 5;
+  const DevView& src = to_office ? home : office;
+  const DevView& dst = to_office ? office : home;
+  for (uint32_t r = ly; r < 32u; r += 8u) {
+    // source rows: home row k = k0 + r (columns j), office row j = j0 + r (columns k)
+    const uint32_t k = to_office ? k0 + r : k0 + lx, j = to_office ? j0 + lx : j0 + r;
+    const uint32_t tk = k - k0, tj = j - j0;
+    if (k < K && j < O) {
+      const uint64_t i = base + (to_office ? static_cast<uint64_t>(k) * O + j : static_cast<uint64_t>(j) * K + k);
+      tile[0][tk][tj] = src.st[i];
+      tile[1][tk][tj] = src.pos[i];
+    }
+  }
+  __syncthreads();
+  for (uint32_t r = ly; r < 32u; r += 8u) {
+    // destination rows: office row j = j0 + r (columns k), home row k = k0 + r (columns j)
+    const uint32_t k = to_office ? k0 + lx : k0 + r, j = to_office ? j0 + r : j0 + lx;
+    const uint32_t tk = k - k0, tj = j - j0;
+    if (k < K && j < O) {
+      const uint64_t i = base + (to_office ? static_cast<uint64_t>(j) * K + k : static_cast<uint64_t>(k) * O + j);
+      dst.st[i] = tile[0][tk][tj];
+      dst.pos[i] = tile[1][tk][tj];
+    }
+  }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -906,6 +1000,13 @@ hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, 
 hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_t stream) {
   if (s.agent_blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(k_apply_sweep, dim3(s.agent_blocks), dim3(s.block), 0, stream, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_relayout(const DevView& home, const DevView& office, const TileRef* tiles, uint32_t n_tiles,
+                           bool to_office, hipStream_t stream) {
+  if (n_tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_relayout, dim3(n_tiles), dim3(256), 0, stream, home, office, tiles, to_office ? 1u : 0u);
   return hipGetLastError();
 }
 

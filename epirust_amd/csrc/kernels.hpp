@@ -10,6 +10,8 @@ namespace epi {
 
 // Device pointers handed to every kernel by value.
 struct DevView {
+  // Agent arrays: pos and st in one layout, home (slot = id) or office (office != 0);
+  // tm and wsa always in the home layout (indexed by id).
   uint32_t* pos;
   uint32_t* st;
   uint32_t* tm;
@@ -26,6 +28,12 @@ struct DevView {
   const uint32_t* vacc_hour;
   const unsigned long long* vacc_thr;
   uint32_t* block_counts;  // 6 per agent block: the hour's partial S E I H R D
+  uint32_t office;         // 1: the agent arrays above are in the office layout
+};
+
+// One tile of a simulation's layout transpose: agents k0..k0+31 of offices j0..j0+31.
+struct TileRef {
+  uint32_t sim, k0, j0;
 };
 
 struct LaunchShape {
@@ -44,6 +52,10 @@ struct LaunchShape {
 hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStream_t stream);
 hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream);
 hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, hipStream_t stream);
+// Copies st and pos between the home and office layouts (to_office: home -> office), one
+// 32 x 32 tile of (agent k, office j) per block.
+hipError_t launch_relayout(const DevView& home, const DevView& office, const TileRef* tiles, uint32_t n_tiles,
+                           bool to_office, hipStream_t stream);
 // Re-applies the last decided sweep of every simulation (idempotent), before agent read-back.
 hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_t stream);
 inline bool hour_has_step(uint32_t hour) { return hour % 24u == 0 || hour % 24u >= 7; }

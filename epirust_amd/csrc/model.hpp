@@ -24,6 +24,9 @@
 
 namespace epi {
 
+// Hours of day whose step and commit run on the office layout of the agent arrays.
+inline constexpr bool office_hour(uint32_t hour) { return hour % 24u >= 8u && hour % 24u <= 15u; }
+
 // ---- state word
 enum : uint32_t {
   ST_S = 0, ST_E = 1, ST_I = 2, ST_R = 3, ST_D = 4,
@@ -106,7 +109,14 @@ struct SimDesc {
   uint32_t has_lockdown, lockdown_at;
   uint32_t has_hospital, spread_rate_threshold;
   uint32_t vacc_off, n_vacc;  // sorted, last-entry-wins schedule in the shared vacc arrays
-  uint32_t n_blocks;          // agent blocks of this simulation (step / commit)
+  // Agent slots. Both layouts of the agent arrays give a simulation n_slots =
+  // n_offices * per_office >= n slots from agent_off. Home layout: slot = id. Office
+  // layout (hours of day 8..15): slot = office * per_office + k holds agent
+  // k * n_offices + office (its office is id % n_offices), so the agents of one office sit
+  // together. Slots whose agent would be >= n are holes in both layouts.
+  uint32_t n_slots, per_office;
+  FastDiv fd_per_office;
+  uint32_t n_blocks;          // agent blocks of this simulation (step / commit), over n_slots
   uint32_t block_begin;       // index of its first agent block
 };
 
