@@ -18,7 +18,7 @@ import json
 import re
 import sys
 
-HOUR_KERNELS = ("k_hospital_scan", "k_step", "k_commit")
+HOUR_KERNELS = ("k_hospital_scan", "k_relayout", "k_step", "k_commit")
 
 
 def short(name):
@@ -32,9 +32,9 @@ def short(name):
 
 def by_hour_of_day(trace_csv, hours):
     """Mean duration per kernel for each hour of day over the first `hours` hours of the
-    trace (an hour is [k_hospital_scan] [k_step] k_commit: hours are counted by k_commit
-    dispatches, which every hour has), with the wall time from one hour's first kernel to
-    the next hour's."""
+    trace (an hour is [k_hospital_scan] [k_relayout] [k_step] k_commit: hours are counted by
+    k_commit dispatches, which every hour has), with the wall time from one hour's first
+    kernel to the next hour's."""
     ks = []
     for r in csv.DictReader(open(trace_csv)):
         k = short(r["Kernel_Name"])
