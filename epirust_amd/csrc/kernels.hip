@@ -719,7 +719,8 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     const uint64_t gid = D.agent_off + local;  // tm and wsa stay in the home layout (by id)
     a.tm_arr = v.tm + gid;
     a.claims = v.claim + D.cell_off;
-    a.precheck = slot >= kResidentAgents;
+    // at office hours the block claim filter below resolves the contended cells instead
+    a.precheck = slot >= kResidentAgents && !v.office;
     const uint32_t w0 = w_stored;
     const bool staff = a.ws() == WS_STAFF;
     a.wsa = staff ? v.wsa[gid] : 0u;
