@@ -18,6 +18,10 @@
 //              change no state but flags and areas (citizen/mod.rs:244-248): the epilogue
 //              repeats the last counts and the agent pass applies the pending sweep, sets
 //              non-staff areas to home at hour of day 1 and resets the claim array at 2.
+//   k_relayout: [hours of day 8 and 16] transposes st and pos between the home layout
+//              (slot = agent id) and the office layout (an office's agents in consecutive
+//              slots) that hours of day 8..15 run on; there k_step first resolves the
+//              claims of each office's cells among the block's agents in LDS.
 // A sweep decided at hour h (allocation_map.rs:306-337) is applied by the agent pass of
 // hour h+1 (before the routine, in k_step), which is what the reference's next hour sees;
 // sweeps are idempotent, so k_apply_sweep re-applies the last one before the host reads
