@@ -723,8 +723,11 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     const uint64_t gid = D.agent_off + local;  // tm and wsa stay in the home layout (by id)
     a.tm_arr = v.tm + gid;
     a.claims = v.claim + D.cell_off;
-    // at office hours the block claim filter below resolves the contended cells instead
-    a.precheck = slot >= kResidentAgents && !v.office;
+    // At office hours the block claim filter below resolves the contended cells instead.
+    // At the other hours a local move stays in the agent's 2x2 house, whose other agents
+    // (ids congruent mod n_houses) are its only rivals: the first of a house (id below
+    // n_houses) has no lower-id rival to find.
+    a.precheck = slot >= kResidentAgents && !v.office && local >= D.n_houses;
     const uint32_t w0 = w_stored;
     const bool staff = a.ws() == WS_STAFF;
     a.wsa = staff ? v.wsa[gid] : 0u;
