@@ -475,6 +475,7 @@ struct Agent {
       ty = hr.y0 + static_cast<int32_t>(fv / wd);
       w |= F_HOSP;
     } else {
+      mv = dr(SITE_MOVE, 0);  // hour of day 0 draws it only here
       int32_t px, py;
       random_point(home(), px, py);
       move_to(px, py);
@@ -704,15 +705,16 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   const bool filter = v.office && (j_hi - j_lo + 1u) * kOfficeCells <= kFilterCells;
   if (valid) {
     // st and pos are loaded together, then the work that needs neither (the buildings and
-    // the hour's movement draw), then one wait (wsa and tm are read by few agents and
-    // loaded where they are needed).
+    // the hour's movement draw, which hour of day 0 does not use: hospitalize draws it for
+    // its rare fallback), then one wait (wsa and tm are read by few agents and loaded where
+    // they are needed).
     uint32_t w_stored = v.st[gi];
     uint32_t p = v.pos[gi];
     Agent a(D, *S, occ);
     a.id = local;
     a.h = h;
     a.locate_buildings();
-    a.mv = a.dr(SITE_MOVE, 0);
+    if (hd != 0) a.mv = a.dr(SITE_MOVE, 0);
     loaded_together(w_stored, p, a.mv);
     a.w = sweep.mask ? apply_sweep(w_stored, sweep, local, D) : w_stored;
     a.x = static_cast<int32_t>(p & 0xFFFFu);
