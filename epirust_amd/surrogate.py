@@ -12,9 +12,13 @@ plain simulation stays the fallback and the measure of its error:
 * peak infected = max over days of the infected fraction; final size = infected +
   hospitalized + recovered + deceased on the last day (the reference's `totalinfected`,
   engine/plot/update_total_infections.py:19);
-* two independently initialised networks are trained; their disagreement on peak and
-  final size is the predicted error, and a point whose predicted error exceeds the
-  threshold is simulated instead (32 seeds, batched).
+* two independently initialised networks are trained on linear (standardized) fractions,
+  so that the fit estimates the mean curve without the downward bias a log or square-root
+  target would give;
+* the predicted error of a point combines the two networks' disagreement on peak and final
+  size with the standard error of the 32-seed ensemble mean it is judged against, estimated
+  from the single-simulation residuals of the nearest training points; a point whose
+  predicted error exceeds the threshold is simulated instead (32 seeds, batched).
 """
 import copy
 import ctypes
@@ -202,17 +206,39 @@ def train_mlp(X, Y, seed, epochs=60, batch=512, lr=1e-3, widths=WIDTHS, log=None
     return m, hist
 
 
-class Surrogate:
-    """Two-network ensemble with standardized I/O and a simulation fallback."""
+class NoiseModel:
+    """Seed-to-seed spread of peak and final size around the network's answer, as a function
+    of the parameters: the root mean square of the training simulations' residuals (single
+    seed minus prediction) over the k nearest training points. Divided by sqrt(seeds) it is
+    the standard error of a seeds-member ensemble mean, the quantity a prediction is judged
+    against."""
 
-    def __init__(self, spec, members, y_mean, y_std):
+    def __init__(self, unit_x, resid_peak, resid_final, k=64):
+        from scipy.spatial import cKDTree
+        self.x = np.asarray(unit_x, dtype=np.float64)
+        self.r = np.stack([np.asarray(resid_peak, np.float64), np.asarray(resid_final, np.float64)], axis=1)
+        self.k = int(min(k, len(self.x)))
+        self.tree = cKDTree(self.x)
+
+    def spread(self, unit_x):
+        """(n, 2) RMS single-simulation residual of peak and final size near each point."""
+        _, nb = self.tree.query(np.asarray(unit_x, dtype=np.float64), k=self.k)
+        nb = nb.reshape(len(unit_x), self.k)
+        return np.sqrt(np.mean(self.r[nb] ** 2, axis=1))
+
+
+class Surrogate:
+    """Two-network ensemble with standardized I/O, a noise model and a simulation fallback."""
+
+    def __init__(self, spec, members, y_mean, y_std, noise=None):
         self.spec = spec
         self.members = members
         self.y_mean = y_mean
         self.y_std = y_std
+        self.noise = noise
 
     @classmethod
-    def fit(cls, unit_x, curves, spec, epochs=60, seeds=(1, 2), log=None):
+    def fit(cls, unit_x, curves, spec, epochs=100, seeds=(1, 2), log=None):
         days = spec["days"]
         Y = curves.reshape(len(curves), days * 6).astype(np.float64)
         y_mean = Y.mean(axis=0)
@@ -223,7 +249,11 @@ class Surrogate:
         for s in seeds:
             m, hist = train_mlp(X, Yn, seed=s, epochs=epochs, log=log)
             members.append((m, hist))
-        return cls(spec, [m for m, _ in members], y_mean, y_std), [h for _, h in members]
+        sur = cls(spec, [m for m, _ in members], y_mean, y_std)
+        fit_p, fit_f = peak_and_final(np.mean(sur.predict_members(X), axis=0))
+        lab_p, lab_f = peak_and_final(curves)
+        sur.noise = NoiseModel(X, lab_p - fit_p, lab_f - fit_f)
+        return sur, [h for _, h in members]
 
     def predict_members(self, unit_x):
         days = self.spec["days"]
@@ -234,14 +264,22 @@ class Surrogate:
             out.append(np.clip(y, 0.0, 1.0).reshape(len(unit_x), days, 6))
         return out
 
-    def predict(self, unit_x):
-        """Mean curves and the predicted relative error (member disagreement) of peak and final size."""
+    def predict(self, unit_x, seeds=None):
+        """Mean curves and the predicted relative error of peak and final size against a
+        `seeds`-member simulation ensemble (default: the evaluation ensemble size): the
+        members' disagreement and the ensemble mean's standard error, added in quadrature."""
         ys = self.predict_members(unit_x)
         mean = np.mean(ys, axis=0)
         pk, fs = zip(*[peak_and_final(y) for y in ys])
         pk, fs = np.array(pk), np.array(fs)
         mp, mf = peak_and_final(mean)
-        err = np.maximum(np.abs(pk[0] - pk[1]) / np.maximum(mp, 1e-9), np.abs(fs[0] - fs[1]) / np.maximum(mf, 1e-9))
+        mp, mf = np.maximum(mp, 1e-9), np.maximum(mf, 1e-9)
+        dis_p, dis_f = np.abs(pk[0] - pk[1]) / mp, np.abs(fs[0] - fs[1]) / mf
+        if self.noise is None:
+            return mean, np.maximum(dis_p, dis_f)
+        seeds = self.spec["eval_seeds"] if seeds is None else seeds
+        sp = self.noise.spread(unit_x) / math.sqrt(seeds)
+        err = np.maximum(np.hypot(dis_p, sp[:, 0] / mp), np.hypot(dis_f, sp[:, 1] / mf))
         return mean, err
 
     def sweep(self, unit_x, seeds_per_point, threshold=None, block=None):
@@ -250,7 +288,7 @@ class Surrogate:
         (curves, fallback mask, seconds)."""
         threshold = self.spec["max_relative_error"] if threshold is None else threshold
         t0 = time.perf_counter()
-        curves, err = self.predict(unit_x)
+        curves, err = self.predict(unit_x, seeds_per_point)
         fb = err > threshold
         idx = np.nonzero(fb)[0]
         if len(idx):
@@ -289,7 +327,7 @@ def relative_errors(pred_curves, ref_curves):
     return np.abs(pp - rp) / np.maximum(rp, 1e-9), np.abs(pf - rf) / np.maximum(rf, 1e-9)
 
 
-def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=60, block=None, log=print):
+def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=100, block=None, log=print):
     """Generate data, train, evaluate; writes surrogate_weights.npz and surrogate_metrics.json."""
     spec = load_spec()
     n_train = n_train or spec["train_points"]
@@ -330,7 +368,7 @@ def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=60,
     metrics["plain_points_per_s"] = n_eval / plain_seconds
 
     t0 = time.perf_counter()
-    pred, err_hat = sur.predict(ex)
+    pred, err_hat = sur.predict(ex, eval_seeds)
     metrics["surrogate_only_seconds"] = time.perf_counter() - t0
     ep_, ef_ = relative_errors(pred, ref)
     nn = nearest_neighbour(ux, curves, ex)
@@ -347,6 +385,16 @@ def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=60,
     sp, sf = peak_and_final(ens)
     metrics["ensemble_cv_peak_median"] = float(np.median(sp.std(axis=1) / np.maximum(sp.mean(axis=1), 1e-9)))
     metrics["ensemble_cv_final_median"] = float(np.median(sf.std(axis=1) / np.maximum(sf.mean(axis=1), 1e-9)))
+    # What the ensemble's own noise allows: half the difference of the two half-ensemble
+    # means has the variance of the full ensemble mean's error around the true mean, so the
+    # fraction of points where it is within the bound estimates the score of a predictor
+    # that knew the true mean curve exactly.
+    h = eval_seeds // 2
+    if h:
+        hp = np.abs(sp[:, :h].mean(axis=1) - sp[:, h:2 * h].mean(axis=1)) / 2 / np.maximum(sp.mean(axis=1), 1e-9)
+        hf = np.abs(sf[:, :h].mean(axis=1) - sf[:, h:2 * h].mean(axis=1)) / 2 / np.maximum(sf.mean(axis=1), 1e-9)
+        metrics["exact_mean_fraction_within_estimate"] = float(
+            ((hp <= spec["max_relative_error"]) & (hf <= spec["max_relative_error"])).mean())
 
     curves_fb, fb, sweep_seconds = sur.sweep(ex, eval_seeds, block=block)
     efp, eff = relative_errors(curves_fb, ref)
@@ -380,7 +428,7 @@ def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=60,
                         rel_err_peak=ep_, rel_err_final=ef_, nn_rel_err_peak=np_, nn_rel_err_final=nf_)
     flat = [m.params() for m in sur.members]
     np.savez_compressed(os.path.join(out_dir, "surrogate_weights.npz"), member0=flat[0], member1=flat[1],
-                        y_mean=sur.y_mean, y_std=sur.y_std)
+                        y_mean=sur.y_mean, y_std=sur.y_std, noise_x=sur.noise.x, noise_resid=sur.noise.r)
     with open(os.path.join(out_dir, "surrogate_metrics.json"), "w") as f:
         json.dump(metrics, f, indent=1)
     log(json.dumps(metrics))
@@ -394,7 +442,7 @@ if __name__ == "__main__":
     ap.add_argument("--train", type=int, default=0)
     ap.add_argument("--eval", type=int, default=0)
     ap.add_argument("--seeds", type=int, default=0)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=100)
     a = ap.parse_args()
     run_pipeline(a.out, a.train or None, a.eval or None, a.seeds or None, a.epochs,
                  log=lambda s: print(s, flush=True))

@@ -76,3 +76,54 @@ def test_relative_errors():
     ep, ef = S.relative_errors(pred, ref)
     assert ep[0] == 0 and ep[1] == pytest.approx(0.1)
     assert ef[1] == pytest.approx(0.02 / 0.5)
+
+
+def test_noise_model_recovers_a_local_spread():
+    # residuals of spread 0.01 where x0 < 0.5 and 0.04 elsewhere (random signs): the RMS over
+    # the nearest training points recovers each side's spread away from the boundary
+    g = np.random.default_rng(3)
+    x = g.random((4000, 2))
+    sd = np.where(x[:, 0] < 0.5, 0.01, 0.04)
+    sign = g.choice([-1.0, 1.0], size=4000)
+    nm = S.NoiseModel(x, sign * sd, 2 * sign * sd, k=64)
+    sp = nm.spread(np.array([[0.1, 0.5], [0.9, 0.5]]))
+    assert sp[0] == pytest.approx([0.01, 0.02])
+    assert sp[1] == pytest.approx([0.04, 0.08])
+    assert S.NoiseModel(x[:10], sd[:10], sd[:10], k=64).k == 10  # k is capped at the training size
+
+
+class _Member:
+    """Stands in for a network: returns fixed standardized outputs."""
+
+    def __init__(self, y):
+        self.y = y
+
+    def forward(self, x):
+        return np.repeat(self.y[None], len(x), axis=0)
+
+
+def test_predicted_error_adds_disagreement_and_ensemble_standard_error():
+    spec = dict(S.load_spec(), days=2)
+    # curves: infected 0.1 then 0.2 (peak 0.2), recovered 0.3 on the last day (final 0.5)
+    c = np.zeros((2, 6))
+    c[:, 2] = [0.1, 0.2]
+    c[1, 4] = 0.3
+    c[:, 0] = 1 - c.sum(axis=1)
+    y = c.reshape(-1)
+    x = np.full((1, 9), 0.5)
+    # identical members, residual spread 0.02 (peak) and 0.04 (final), 16 seeds: standard
+    # errors 0.005 and 0.01, relative 0.025 and 0.02 -> predicted error 0.025
+    nm = S.NoiseModel(np.random.default_rng(0).random((100, 9)), np.full(100, 0.02), np.full(100, -0.04))
+    sur = S.Surrogate(spec, [_Member(np.zeros(12)), _Member(np.zeros(12))], y, np.ones(12), nm)
+    mean, err = sur.predict(x, seeds=16)
+    assert np.allclose(mean[0], c)
+    assert err[0] == pytest.approx(0.025)
+    # members at 0.2 -/+ 0.03 on the peak day: 0.06 apart, relative 0.3 of the mean peak 0.2
+    d = np.zeros(12)
+    d[1 * 6 + 2] = 0.03  # day 1 infected
+    sur2 = S.Surrogate(spec, [_Member(d), _Member(-d)], y, np.ones(12), nm)
+    _, err2 = sur2.predict(x, seeds=16)
+    assert err2[0] == pytest.approx(np.hypot(0.06 / 0.2, 0.025))
+    # without a noise model the error is the disagreement alone
+    _, err3 = S.Surrogate(spec, [_Member(d), _Member(-d)], y, np.ones(12)).predict(x)
+    assert err3[0] == pytest.approx(0.06 / 0.2)
