@@ -21,7 +21,8 @@
 //   k_relayout: [hours of day 8 and 16] transposes st and pos between the home layout
 //              (slot = agent id) and the office layout (an office's agents in consecutive
 //              slots) that hours of day 8..15 run on; there k_step first resolves the
-//              claims of each office's cells among the block's agents in LDS.
+//              claims of each office's cells among the block's agents in LDS, and the
+//              winners on offices that lie wholly in one block claim nothing globally.
 // A sweep decided at hour h (allocation_map.rs:306-337) is applied by the agent pass of
 // hour h+1 (before the routine, in k_step), which is what the reference's next hour sees;
 // sweeps are idempotent, so k_apply_sweep re-applies the last one before the host reads
@@ -47,6 +48,8 @@ constexpr uint32_t kResidentAgents = 256u * 2048u;
 // Office-hour block claim filter: 100 cells per 10 x 10 office, at most kFilterCells words
 // of LDS (a block of up to 1024 slots covers at most 20 offices when per_office >= 52).
 constexpr uint32_t kOfficeCells = 100u, kFilterCells = 2048u, kNoCell = ~0u;
+// Target word flag (y never exceeds 32000, so bit 31 is free): the move has already won.
+constexpr uint32_t kTgtWon = 1u << 31;
 
 // Agent id of a slot: the identity in the home layout; slot = office * per_office + k
 // holds agent k * n_offices + office in the office layout.
@@ -695,6 +698,7 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   // This lane's claim and state word, carried past the block's claim filter to the stores.
   bool moves = false;
   uint32_t key = 0, w_out = 0, w_in = 0, tgt_word = 0, office_cell = kNoCell;
+  bool whole_office = false;  // the claimed cell's office has all its slots in this block
   const uint64_t gi = D.agent_off + slot;
   uint64_t ci = 0;
   // Office hours: the offices this block's slots belong to (j_lo .. j_lo + n_off - 1), each
@@ -771,9 +775,11 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     // cells at these hours) enters the filter
     const uint32_t j = fast_mod(local, D.fd_offices);  // == slot / per_office in the office layout
     if (filter && moves && a.tx >= a.ox && a.tx <= a.ox + 9 && a.ty >= a.oy && a.ty <= a.oy + 9 && j >= j_lo &&
-        j <= j_hi)
+        j <= j_hi) {
       office_cell = (j - j_lo) * kOfficeCells + static_cast<uint32_t>(a.ty - a.oy) * 10u +
                     static_cast<uint32_t>(a.tx - a.ox);
+      whole_office = j * D.per_office >= first && (j + 1u) * D.per_office <= first + blockDim.x;
+    }
     if (tm_dirty) v.tm[gid] = a.tm;
     if (staff && a.wsa != wsa0) v.wsa[gid] = a.wsa;
   }
@@ -781,7 +787,9 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   // or two blocks, and they are the only claimants of its cells, so most same-cell rivals
   // meet here: an LDS atomicMin per claimed cell leaves only the block's lowest key, and a
   // lane that is not it has lost to a lower id and makes no global claim. At most one
-  // global atomic per cell and block remains, instead of one per rival.
+  // global atomic per cell and block remains, instead of one per rival. On an office whose
+  // slots all lie in this block the filter has seen every claimant, so its survivor has
+  // won: it makes no global claim and flags its target word for the commit.
   if (filter) {  // uniform
     __shared__ uint32_t s_cell[kFilterCells];
     const uint32_t n_cells = (j_hi - j_lo + 1u) * kOfficeCells;
@@ -794,7 +802,10 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   if (valid) {
     if (moves) {
       w_out |= F_MOVE;
-      atomicMin(&v.claim[ci], key);
+      if (whole_office)
+        tgt_word |= kTgtWon;
+      else
+        atomicMin(&v.claim[ci], key);
       v.tgt[gi] = tgt_word;
     }
     if (w_out != w_in) v.st[gi] = w_out;
@@ -874,14 +885,16 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
     uint32_t p = v.pos[gi];
     uint32_t t = v.tgt[gi];
     loaded_together(w, p, t);
+    const bool won = (t & kTgtWon) != 0u;  // decided by the office-hour claim filter
     if (w & (F_MOVE | F_DIRTY)) {
       uint8_t* occ = v.occ + D.cell_off;
       const uint64_t old_ci = cell_index(D, static_cast<int32_t>(p & 0xFFFFu), static_cast<int32_t>(p >> 16));
       const uint8_t byte = grid_byte(D, w);
       bool moved = false;
       if (w & F_MOVE) {
+        t &= ~kTgtWon;
         const uint64_t ci = cell_index(D, static_cast<int32_t>(t & 0xFFFFu), static_cast<int32_t>(t >> 16));
-        if (v.claim[D.cell_off + ci] == claim_key(h, local)) {
+        if (won || v.claim[D.cell_off + ci] == claim_key(h, local)) {
           occ[old_ci] = 0;
           occ[ci] = byte;
           v.pos[gi] = t;
