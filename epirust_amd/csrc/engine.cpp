@@ -130,7 +130,7 @@ struct epi_sim {
   bool any_empty = false;
   uint32_t flags = 0;
   LaunchShape shape;
-  std::vector<BlockRef> agent_blocks, hosp_blocks;
+  std::vector<BlockRef> agent_blocks, office_blocks, hosp_blocks;
   std::vector<uint32_t> vacc_hour;
   std::vector<unsigned long long> vacc_thr;
   DeviceBuffers mem;
@@ -139,6 +139,7 @@ struct epi_sim {
   TileRef* d_tiles = nullptr;  // layout transpose tiles of every simulation
   uint32_t n_tiles = 0;
   BlockRef* d_agent_blocks = nullptr;
+  BlockRef* d_office_blocks = nullptr;
   SimDesc* d_desc = nullptr;
   hipStream_t stream = nullptr;
   hipGraphExec_t day_graph = nullptr;
@@ -214,6 +215,23 @@ int build_launch_tables(epi_sim* s) {
     for (uint32_t b = 0; b < nb; ++b) s->agent_blocks.push_back(BlockRef{i, b * B});
   }
   s->shape.agent_blocks = static_cast<uint32_t>(s->agent_blocks.size());
+  // Office layout: floor(B / K) whole offices of K slots per block; n_slots is a multiple
+  // of K, so no office is split. Idle lanes cost latency hiding (at 256 threads and 139
+  // slots per office, 10M agents ran 14% slower), so this needs blocks at least 7/8 full;
+  // otherwise blocks of B consecutive slots. Either way a block has more than B / 2 slots,
+  // so there is at most one block per 32 slots.
+  s->office_blocks.clear();
+  for (uint32_t i = 0; i < s->n_sims; ++i) {
+    SimDesc& D = s->desc[i];
+    const uint32_t K = D.per_office;
+    const uint32_t whole = (K >= 1 && K <= B) ? (B / K) * K : 0u;
+    D.office_span = 8u * whole >= 7u * B ? whole : B;
+    const uint32_t nb = (D.n_slots + D.office_span - 1) / D.office_span;
+    D.office_n_blocks = nb;
+    D.office_block_begin = static_cast<uint32_t>(s->office_blocks.size());
+    for (uint32_t b = 0; b < nb; ++b) s->office_blocks.push_back(BlockRef{i, b * D.office_span});
+  }
+  s->shape.office_agent_blocks = static_cast<uint32_t>(s->office_blocks.size());
   return EPI_OK;
 }
 
@@ -221,6 +239,9 @@ int build_launch_tables(epi_sim* s) {
 int upload_tables(epi_sim* s) {
   if (s->shape.agent_blocks)
     HIP_TRY(hipMemcpyAsync(s->d_agent_blocks, s->agent_blocks.data(), s->agent_blocks.size() * sizeof(BlockRef),
+                           hipMemcpyHostToDevice, s->stream));
+  if (s->shape.office_agent_blocks)
+    HIP_TRY(hipMemcpyAsync(s->d_office_blocks, s->office_blocks.data(), s->office_blocks.size() * sizeof(BlockRef),
                            hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(s->d_desc, s->desc.data(), s->desc.size() * sizeof(SimDesc), hipMemcpyHostToDevice,
                          s->stream));
@@ -595,11 +616,16 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   HIP_TRY(M.alloc(&s->d_desc, s->n_sims));
   v.desc = s->d_desc;
   HIP_TRY(M.alloc(&v.state, s->n_sims));
-  // table sized for the smallest legal block (64 threads) so block-size changes never reallocate
-  uint64_t max_blocks = 0;
-  for (uint32_t i = 0; i < s->n_sims; ++i) max_blocks += (s->desc[i].n_slots + 63) / 64;
+  // tables sized for the smallest legal block (64 threads; office blocks of at least 33
+  // slots) so block-size changes never reallocate
+  uint64_t max_blocks = 0, max_office_blocks = 0;
+  for (uint32_t i = 0; i < s->n_sims; ++i) {
+    max_blocks += (s->desc[i].n_slots + 63) / 64;
+    max_office_blocks += (s->desc[i].n_slots + 31) / 32;
+  }
   HIP_TRY(M.alloc(&s->d_agent_blocks, max_blocks));
-  HIP_TRY(M.alloc(&v.block_counts, 6 * max_blocks));
+  HIP_TRY(M.alloc(&s->d_office_blocks, max_office_blocks));
+  HIP_TRY(M.alloc(&v.block_counts, 6 * max_office_blocks));
   v.agent_blocks = s->d_agent_blocks;
   BlockRef* d_hosp_blocks = nullptr;
   HIP_TRY(M.alloc(&d_hosp_blocks, s->hosp_blocks.size()));
@@ -661,6 +687,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     DevView o = v;
     o.pos = vo.pos;
     o.st = vo.st;
+    o.agent_blocks = s->d_office_blocks;
     o.office = 1;
     v.office = 0;
     vo = o;

@@ -20,9 +20,10 @@
 //              non-staff areas to home at hour of day 1 and resets the claim array at 2.
 //   k_relayout: [hours of day 8 and 16] transposes st and pos between the home layout
 //              (slot = agent id) and the office layout (an office's agents in consecutive
-//              slots) that hours of day 8..15 run on; there k_step first resolves the
-//              claims of each office's cells among the block's agents in LDS, and the
-//              winners on offices that lie wholly in one block claim nothing globally.
+//              slots) that hours of day 8..15 run on, with blocks of whole offices;
+//              there k_step resolves the claims of each office's cells among the
+//              block's agents in LDS, and the winners on offices that lie wholly in one
+//              block (all of them when an office fits a block) claim nothing globally.
 // A sweep decided at hour h (allocation_map.rs:306-337) is applied by the agent pass of
 // hour h+1 (before the routine, in k_step), which is what the reference's next hour sees;
 // sweeps are idempotent, so k_apply_sweep re-applies the last one before the host reads
@@ -660,8 +661,11 @@ __device__ void run_epilogue(const DevView& v, uint32_t sim, const SimDesc& D, S
   if (threadIdx.x < 6) s_sum[threadIdx.x] = 0;
   __syncthreads();
   uint32_t acc[6] = {0, 0, 0, 0, 0, 0};
-  for (uint32_t b = threadIdx.x; b < D.n_blocks; b += blockDim.x) {
-    const uint32_t* bc = v.block_counts + 6u * (D.block_begin + b);
+  // the partials of the block table the hour ran on
+  const uint32_t n_blocks = v.office ? D.office_n_blocks : D.n_blocks;
+  const uint32_t begin = v.office ? D.office_block_begin : D.block_begin;
+  for (uint32_t b = threadIdx.x; b < n_blocks; b += blockDim.x) {
+    const uint32_t* bc = v.block_counts + 6u * (begin + b);
 #pragma unroll
     for (int k = 0; k < 6; ++k) acc[k] += bc[k];
   }
@@ -691,7 +695,8 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   const Sweep sweep = sweep_slot(*S, (h - 1u) & 1u);  // decided at hour h - 1
   const uint32_t slot = first_lane(bm.first) + threadIdx.x;
   const uint32_t local = agent_of_slot(v, D, slot);  // the agent id
-  const bool valid = slot < D.n_slots && local < D.n;
+  const uint32_t span = v.office ? D.office_span : blockDim.x;  // slots of this block
+  const bool valid = slot < D.n_slots && local < D.n && threadIdx.x < span;
   const uint8_t* occ = v.occ + D.cell_off;
 
   uint32_t cat = 6;
@@ -705,7 +710,7 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   // with 100 cells in the block's claim filter; computed before any store.
   const uint32_t first = first_lane(bm.first);
   const uint32_t j_lo = v.office ? fast_div(first, D.fd_per_office) : 0u;
-  const uint32_t j_hi = v.office ? min(D.n_offices - 1u, fast_div(first + blockDim.x - 1u, D.fd_per_office)) : 0u;
+  const uint32_t j_hi = v.office ? min(D.n_offices - 1u, fast_div(first + span - 1u, D.fd_per_office)) : 0u;
   const bool filter = v.office && (j_hi - j_lo + 1u) * kOfficeCells <= kFilterCells;
   if (valid) {
     // st and pos are loaded together, then the work that needs neither (the buildings and
@@ -778,7 +783,7 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
         j <= j_hi) {
       office_cell = (j - j_lo) * kOfficeCells + static_cast<uint32_t>(a.ty - a.oy) * 10u +
                     static_cast<uint32_t>(a.tx - a.ox);
-      whole_office = j * D.per_office >= first && (j + 1u) * D.per_office <= first + blockDim.x;
+      whole_office = j * D.per_office >= first && (j + 1u) * D.per_office <= first + span;
     }
     if (tm_dirty) v.tm[gid] = a.tm;
     if (staff && a.wsa != wsa0) v.wsa[gid] = a.wsa;
@@ -877,7 +882,7 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
     return;
   }
   const uint32_t local = agent_of_slot(v, D, slot);  // the agent id
-  if (slot < D.n_slots && local < D.n) {
+  if (slot < D.n_slots && local < D.n && (!v.office || threadIdx.x < D.office_span)) {
     const uint64_t gi = D.agent_off + slot;
     // st, pos and tgt are loaded together so a mover pays two dependent round trips
     // (these three, then its claim), not four.
@@ -1009,14 +1014,16 @@ hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStrea
 }
 
 hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream) {
-  if (s.agent_blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_step, dim3(s.agent_blocks), dim3(s.block), 0, stream, v);
+  const uint32_t blocks = v.office ? s.office_agent_blocks : s.agent_blocks;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_step, dim3(blocks), dim3(s.block), 0, stream, v);
   return hipGetLastError();
 }
 
 hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, hipStream_t stream) {
-  if (s.agent_blocks + s.n_sims == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_commit, dim3(s.n_sims + s.agent_blocks), dim3(s.block), 0, stream, v, hour & 1u, s.n_sims);
+  const uint32_t blocks = v.office ? s.office_agent_blocks : s.agent_blocks;
+  if (blocks + s.n_sims == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_commit, dim3(s.n_sims + blocks), dim3(s.block), 0, stream, v, hour & 1u, s.n_sims);
   return hipGetLastError();
 }
 

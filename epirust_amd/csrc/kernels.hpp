@@ -21,7 +21,7 @@ struct DevView {
   uint32_t* claim;  // per cell: lowest claim key of the day's latest claim hour, ~0u = none
   const SimDesc* desc;
   SimState* state;
-  const BlockRef* agent_blocks;  // per agent block: (sim, first local agent)
+  const BlockRef* agent_blocks;  // per agent block: (sim, first slot); the office view has its own
   const BlockRef* hosp_blocks;   // per hospital-scan block: (sim, first cell)
   Counts7* rows;
   Event* events;
@@ -38,7 +38,8 @@ struct TileRef {
 
 struct LaunchShape {
   uint32_t block = 256;          // threads per agent block
-  uint32_t agent_blocks = 0;     // total agent blocks over the batch
+  uint32_t agent_blocks = 0;     // total agent blocks over the batch (home layout)
+  uint32_t office_agent_blocks = 0;  // the same for the office layout's table
   uint32_t hosp_block = 256;     // threads per hospital-scan block
   uint32_t hosp_cells_per_thread = 8;
   uint32_t hosp_blocks = 0;

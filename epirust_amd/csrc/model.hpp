@@ -118,6 +118,10 @@ struct SimDesc {
   FastDiv fd_per_office;
   uint32_t n_blocks;          // agent blocks of this simulation (step / commit), over n_slots
   uint32_t block_begin;       // index of its first agent block
+  // Office-layout hours use a table of their own: blocks of office_span slots, a whole
+  // number of offices when per_office <= the block size (else the block size), so that
+  // each office's slots lie in one block.
+  uint32_t office_span, office_n_blocks, office_block_begin;
 };
 
 // Mutable per-simulation scalars, owned by the device.
