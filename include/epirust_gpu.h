@@ -146,7 +146,8 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
 int epi_last_run_ms(epi_sim* sim, double* device_ms, uint32_t* hours);
 
 /* Launch shape: threads per agent block (64..1024, multiple of 64). Takes effect on the
- * next step; discards the captured day graph. */
+ * next step; discards the captured day graph. Default: 512 for a single simulation of at
+ * least 2^19 agents, else 256. */
 int epi_set_block_size(epi_sim* sim, uint32_t threads);
 
 /* Message of the last failing call on this thread ("" if none). */
