@@ -140,6 +140,9 @@ struct epi_sim {
   uint32_t n_tiles = 0;
   BlockRef* d_agent_blocks = nullptr;
   BlockRef* d_office_blocks = nullptr;
+  // Day graphs run hours of day 3..6 as one epilogue-only launch: no simulation has a
+  // vaccination at an hour of day 2..5, so no sweep is pending at 3..6.
+  bool quiet_hours = false;
   SimDesc* d_desc = nullptr;
   hipStream_t stream = nullptr;
   hipGraphExec_t day_graph = nullptr;
@@ -285,7 +288,14 @@ int capture_day_graph(epi_sim* s) {
   hipGraph_t graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = EPI_OK;
-  for (uint32_t k = 1; k <= 24 && rc == EPI_OK; ++k) rc = enqueue_hour(s, k);  // hours 24d+1 .. 24d+24
+  for (uint32_t k = 1; k <= 24 && rc == EPI_OK; ++k) {  // hours 24d+1 .. 24d+24
+    if (s->quiet_hours && k == 3) {  // hours of day 3..6: epilogues only, one launch
+      if (launch_quiet_hours(s->v, s->shape, 4, s->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      k = 6;
+      continue;
+    }
+    rc = enqueue_hour(s, k);
+  }
   hipError_t e = hipStreamEndCapture(s->stream, &graph);
   if (rc != EPI_OK) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -600,6 +610,9 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   // least 2^19 agents (1M agents: 31 against 33 µs per hour at 256; 10M: 231 against 250),
   // 256 for batches and smaller runs. epi_set_block_size overrides it.
   s->shape.block = (s->n_sims == 1 && s->desc[0].n >= (1u << 19)) ? 512u : 256u;
+  s->quiet_hours = true;
+  for (uint32_t h : s->vacc_hour)
+    if (h % 24u >= 2u && h % 24u <= 5u) s->quiet_hours = false;
   rc = build_launch_tables(s.get());
   if (rc != EPI_OK) return rc;
 

@@ -842,6 +842,29 @@ __device__ __forceinline__ void epilogue_block(const DevView& v, uint32_t sim) {
   run_epilogue(v, sim, v.desc[sim], S, S->hour + 1);
 }
 
+// Epilogues of hours of day 3..6 in one launch (one block per simulation, thread 0 walks
+// the hours), for day graphs of simulations with no vaccination at hours of day 2..5. Then
+// no sweep is pending at any of those hours (lockdown, unlock and hospital decisions fall
+// on hours of day 0 only), so their agent passes would change nothing. Each hour does
+// what epilogue_block does for it.
+__global__ void k_quiet_hours(DevView v, uint32_t n_hours) {
+  if (threadIdx.x != 0) return;
+  const uint32_t sim = blockIdx.x;
+  SimState* S = v.state + sim;
+  for (uint32_t k = 0; k < n_hours; ++k) {
+    if (!S->active) {
+      S->commit_hour[S->hour & 1u] = 0u;
+      continue;
+    }
+    EpilogueIn in;
+    load_epilogue_in(v, sim, in);
+    uint32_t c[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) c[q] = in.st.last_counts[q];
+    hour_epilogue(v, in, S, in.st.hour + 1u, c);
+  }
+}
+
 // Agent pass of an hour of day 1..6 (block b of the simulation's agent blocks): nobody
 // moves and no disease update runs while asleep (citizen/mod.rs:244-248), so only the
 // sweep decided at hour h - 1 is applied, hour of day 1 sets every non-staff current area
@@ -1024,6 +1047,12 @@ hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, 
   const uint32_t blocks = v.office ? s.office_agent_blocks : s.agent_blocks;
   if (blocks + s.n_sims == 0) return hipSuccess;
   hipLaunchKernelGGL(k_commit, dim3(s.n_sims + blocks), dim3(s.block), 0, stream, v, hour & 1u, s.n_sims);
+  return hipGetLastError();
+}
+
+hipError_t launch_quiet_hours(const DevView& v, const LaunchShape& s, uint32_t n_hours, hipStream_t stream) {
+  if (s.n_sims == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_quiet_hours, dim3(s.n_sims), dim3(64), 0, stream, v, n_hours);
   return hipGetLastError();
 }
 

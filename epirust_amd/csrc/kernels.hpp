@@ -57,6 +57,9 @@ hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, 
 // 32 x 32 tile of (agent k, office j) per block.
 hipError_t launch_relayout(const DevView& home, const DevView& office, const TileRef* tiles, uint32_t n_tiles,
                            bool to_office, hipStream_t stream);
+// The epilogues of n_hours consecutive hours of day 3..6 in one launch, with no agent pass
+// (only valid when no sweep can be pending at those hours: no vaccination at 2..5).
+hipError_t launch_quiet_hours(const DevView& v, const LaunchShape& s, uint32_t n_hours, hipStream_t stream);
 // Re-applies the last decided sweep of every simulation (idempotent), before agent read-back.
 hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_t stream);
 inline bool hour_has_step(uint32_t hour) { return hour % 24u == 0 || hour % 24u >= 7; }
