@@ -18,7 +18,8 @@ import json
 import re
 import sys
 
-HOUR_KERNELS = ("k_hospital_scan", "k_relayout", "k_step", "k_commit")
+HOUR_KERNELS = ("k_hospital_scan", "k_relayout", "k_step", "k_commit", "k_quiet_hours")
+QUIET_HOURS = 4  # a k_quiet_hours dispatch runs hours of day 3..6
 
 
 def short(name):
@@ -33,8 +34,9 @@ def short(name):
 def by_hour_of_day(trace_csv, hours):
     """Mean duration per kernel for each hour of day over the first `hours` hours of the
     trace (an hour is [k_hospital_scan] [k_relayout] [k_step] k_commit: hours are counted by
-    k_commit dispatches, which every hour has), with the wall time from one hour's first
-    kernel to the next hour's."""
+    k_commit dispatches, and a k_quiet_hours dispatch stands for the four hours of day
+    3..6, reported at 3), with the wall time from one hour's first kernel to the next
+    hour's (a quiet dispatch's wall time is split evenly over its four hours)."""
     ks = []
     for r in csv.DictReader(open(trace_csv)):
         k = short(r["Kernel_Name"])
@@ -42,7 +44,7 @@ def by_hour_of_day(trace_csv, hours):
             ks.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), k))
     ks.sort()
     dur = collections.defaultdict(lambda: collections.defaultdict(list))
-    start = {}
+    start, span = {}, {}
     hour = 1
     for s, e, k in ks:
         if hour > hours:
@@ -50,11 +52,20 @@ def by_hour_of_day(trace_csv, hours):
         start.setdefault(hour, s)
         dur[hour % 24][k].append((e - s) / 1e3)
         if k == "k_commit":
+            span[hour] = 1
             hour += 1
+        elif k == "k_quiet_hours":
+            span[hour] = QUIET_HOURS
+            for q in range(1, QUIET_HOURS):
+                for kk in HOUR_KERNELS:
+                    dur[(hour + q) % 24][kk]  # listed hours, no dispatches
+            hour += QUIET_HOURS
     wall = collections.defaultdict(list)
-    for h in range(1, hours):
-        if h + 1 in start:
-            wall[h % 24].append((start[h + 1] - start[h]) / 1e3)
+    for h in sorted(start):
+        n = span.get(h, 1)
+        if h + n in start and h + n <= hours:
+            for q in range(n):
+                wall[(h + q) % 24].append((start[h + n] - start[h]) / 1e3 / n)
     out = ["| hour of day | " + " | ".join(HOUR_KERNELS) + " | kernels µs | hour-to-hour µs |",
            "|---" * (len(HOUR_KERNELS) + 3) + "|"]
     tot = 0.0
