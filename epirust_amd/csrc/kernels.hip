@@ -704,6 +704,7 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   bool moves = false;
   uint32_t key = 0, w_out = 0, w_in = 0, tgt_word = 0, office_cell = kNoCell;
   bool whole_office = false;  // the claimed cell's office has all its slots in this block
+  bool first_of_house = false;  // the claim is of the agent's own house by its first agent
   const uint64_t gi = D.agent_off + slot;
   uint64_t ci = 0;
   // Office hours: the offices this block's slots belong to (j_lo .. j_lo + n_off - 1), each
@@ -785,6 +786,12 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
                     static_cast<uint32_t>(a.tx - a.ox);
       whole_office = j * D.per_office >= first && (j + 1u) * D.per_office <= first + span;
     }
+    // Home-layout hours: a house's cells are claimed only by its own agents (ids congruent
+    // mod n_houses), and the first of them (id below n_houses) has the lowest id, so its
+    // claim of a cell of its own house wins. It still claims, so that its housemates lose,
+    // but flags its target word so that the commit need not read the claim back.
+    first_of_house = !v.office && local < D.n_houses && a.tx >= a.hx && a.tx <= a.hx + 1 && a.ty >= a.hy &&
+                     a.ty <= a.hy + 1;
     if (tm_dirty) v.tm[gid] = a.tm;
     if (staff && a.wsa != wsa0) v.wsa[gid] = a.wsa;
   }
@@ -807,10 +814,12 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   if (valid) {
     if (moves) {
       w_out |= F_MOVE;
-      if (whole_office)
+      if (whole_office) {
         tgt_word |= kTgtWon;
-      else
+      } else {
         atomicMin(&v.claim[ci], key);
+        if (first_of_house) tgt_word |= kTgtWon;
+      }
       v.tgt[gi] = tgt_word;
     }
     if (w_out != w_in) v.st[gi] = w_out;
