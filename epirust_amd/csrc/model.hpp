@@ -119,8 +119,8 @@ struct SimDesc {
   uint32_t n_blocks;          // agent blocks of this simulation (step / commit), over n_slots
   uint32_t block_begin;       // index of its first agent block
   // Office-layout hours use a table of their own: blocks of office_span slots, a whole
-  // number of offices when per_office <= the block size (else the block size), so that
-  // each office's slots lie in one block.
+  // number of offices (so that each office's slots lie in one block) when that fills at
+  // least 7/8 of the block, else the block size.
   uint32_t office_span, office_n_blocks, office_block_begin;
 };
 
