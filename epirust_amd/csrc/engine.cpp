@@ -143,6 +143,11 @@ struct epi_sim {
   // Day graphs run hours of day 3..6 as one epilogue-only launch: no simulation has a
   // vaccination at an hour of day 2..5, so no sweep is pending at 3..6.
   bool quiet_hours = false;
+  // ... and hours of day 1..6 as one launch when none has one at 1..5 either, and no
+  // simulation's last hour falls at 1..6 (a batch's day graph may hold a shorter run's
+  // last hour): then no simulation stops inside the launch, whose agent blocks read the
+  // stop flag while its epilogues run.
+  bool night = false;
   SimDesc* d_desc = nullptr;
   hipStream_t stream = nullptr;
   hipGraphExec_t day_graph = nullptr;
@@ -289,6 +294,11 @@ int capture_day_graph(epi_sim* s) {
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = EPI_OK;
   for (uint32_t k = 1; k <= 24 && rc == EPI_OK; ++k) {  // hours 24d+1 .. 24d+24
+    if (s->night && k == 1) {  // hours of day 1..6: one launch
+      if (launch_night(s->v, s->shape, s->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
+      k = 6;
+      continue;
+    }
     if (s->quiet_hours && k == 3) {  // hours of day 3..6: epilogues only, one launch
       if (launch_quiet_hours(s->v, s->shape, 4, s->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
       k = 6;
@@ -611,8 +621,15 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   // 256 for batches and smaller runs. epi_set_block_size overrides it.
   s->shape.block = (s->n_sims == 1 && s->desc[0].n >= (1u << 19)) ? 512u : 256u;
   s->quiet_hours = true;
-  for (uint32_t h : s->vacc_hour)
+  s->night = true;
+  for (uint32_t h : s->vacc_hour) {
     if (h % 24u >= 2u && h % 24u <= 5u) s->quiet_hours = false;
+    if (h % 24u >= 1u && h % 24u <= 5u) s->night = false;
+  }
+  for (uint32_t i = 0; i < s->n_sims; ++i) {
+    const uint32_t last = s->desc[i].hours > 0 ? s->desc[i].hours - 1u : 0u;
+    if (last % 24u >= 1u && last % 24u <= 6u) s->night = false;
+  }
   rc = build_launch_tables(s.get());
   if (rc != EPI_OK) return rc;
 

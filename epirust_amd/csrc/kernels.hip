@@ -605,6 +605,11 @@ __device__ void hour_epilogue(const DevView& v, EpilogueIn& in, SimState* S, uin
   // Slots of this hour's parity: the sweep (read by the next hour's agent pass) and the
   // next hour's agent pass (0 once stopped). Constant indices keep st in registers.
   const uint32_t next = st.active ? h + 1u : 0u;
+  if (h % 24u == 0) {
+    st.night_sweep = sweep;
+    st.night_vacc_hour = h;
+    st.night_vacc_thr = vacc_thr;
+  }
   if (h & 1u) {
     st.sweep[1] = sweep;
     st.sweep_vacc_hour[1] = h;
@@ -856,9 +861,7 @@ __device__ __forceinline__ void epilogue_block(const DevView& v, uint32_t sim) {
 // no sweep is pending at any of those hours (lockdown, unlock and hospital decisions fall
 // on hours of day 0 only), so their agent passes would change nothing. Each hour does
 // what epilogue_block does for it.
-__global__ void k_quiet_hours(DevView v, uint32_t n_hours) {
-  if (threadIdx.x != 0) return;
-  const uint32_t sim = blockIdx.x;
+__device__ void quiet_epilogues(const DevView& v, uint32_t sim, uint32_t n_hours) {
   SimState* S = v.state + sim;
   for (uint32_t k = 0; k < n_hours; ++k) {
     if (!S->active) {
@@ -872,6 +875,41 @@ __global__ void k_quiet_hours(DevView v, uint32_t n_hours) {
     for (int q = 0; q < 6; ++q) c[q] = in.st.last_counts[q];
     hour_epilogue(v, in, S, in.st.hour + 1u, c);
   }
+}
+__global__ void k_quiet_hours(DevView v, uint32_t n_hours) {
+  if (threadIdx.x == 0) quiet_epilogues(v, blockIdx.x, n_hours);
+}
+
+// Hours of day 1..6 in one launch, for day graphs of simulations with no vaccination at
+// hours of day 1..5 (so the only sweep pending in the night is hour of day 0's). Blocks
+// 0 .. n_epi-1 walk the six epilogues of simulation blockIdx.x; the agent blocks (home
+// layout) do the night's agent work: hour of day 0's sweep, every non-staff current area
+// to home (hour of day 1), and their share of the daily claim reset (hour of day 2). The
+// epilogues rewrite the parity slots meanwhile, so the agent blocks read hour of day 0's
+// sweep from its own slot, and whether the simulation runs from `active`, which these
+// hours cannot change (counts repeat, and a day graph never holds the last hour).
+__global__ void k_night(DevView v, uint32_t n_epi) {
+  if (blockIdx.x < n_epi) {  // uniform
+    if (threadIdx.x == 0) quiet_epilogues(v, blockIdx.x, 6);
+    return;
+  }
+  const uint32_t b = blockIdx.x - n_epi;
+  const BlockRef bm = v.agent_blocks[b];
+  const uint32_t sim = first_lane(bm.sim);
+  const SimDesc& D = v.desc[sim];
+  const SimState* S = v.state + sim;
+  if (!S->active) return;  // uniform
+  const Sweep sweep{S->night_sweep, S->night_vacc_hour, S->night_vacc_thr};
+  const ClaimShare share = claim_share(v, D, b - D.block_begin);
+  const uint32_t local = first_lane(bm.first) + threadIdx.x;
+  if (local < D.n) {
+    const uint64_t gi = D.agent_off + local;
+    const uint32_t w = v.st[gi];
+    uint32_t nw = sweep.mask ? apply_sweep(w, sweep, local, D) : w;
+    if (fld(nw, B_WS, M_WS) != WS_STAFF) nw = set_fld(nw, B_AREA, M_AREA, AREA_HOME);
+    if (nw != w) v.st[gi] = nw;
+  }
+  reset_claims(share);
 }
 
 // Agent pass of an hour of day 1..6 (block b of the simulation's agent blocks): nobody
@@ -1056,6 +1094,12 @@ hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, 
   const uint32_t blocks = v.office ? s.office_agent_blocks : s.agent_blocks;
   if (blocks + s.n_sims == 0) return hipSuccess;
   hipLaunchKernelGGL(k_commit, dim3(s.n_sims + blocks), dim3(s.block), 0, stream, v, hour & 1u, s.n_sims);
+  return hipGetLastError();
+}
+
+hipError_t launch_night(const DevView& v, const LaunchShape& s, hipStream_t stream) {
+  if (s.agent_blocks + s.n_sims == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_night, dim3(s.n_sims + s.agent_blocks), dim3(s.block), 0, stream, v, s.n_sims);
   return hipGetLastError();
 }
 

@@ -60,6 +60,10 @@ hipError_t launch_relayout(const DevView& home, const DevView& office, const Til
 // The epilogues of n_hours consecutive hours of day 3..6 in one launch, with no agent pass
 // (only valid when no sweep can be pending at those hours: no vaccination at 2..5).
 hipError_t launch_quiet_hours(const DevView& v, const LaunchShape& s, uint32_t n_hours, hipStream_t stream);
+// Hours of day 1..6 in one launch (home layout): their epilogues, and the night's agent work
+// (hour of day 0's sweep, areas to home, claim reset). Only valid when no vaccination falls
+// at hours of day 1..5.
+hipError_t launch_night(const DevView& v, const LaunchShape& s, hipStream_t stream);
 // Re-applies the last decided sweep of every simulation (idempotent), before agent read-back.
 hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_t stream);
 inline bool hour_has_step(uint32_t hour) { return hour % 24u == 0 || hour % 24u >= 7; }

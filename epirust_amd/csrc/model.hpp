@@ -143,6 +143,10 @@ struct SimState {
   uint32_t last_counts[6];  // S E I H R D of the last row (the population's before hour 1)
   uint32_t rows_written;
   uint32_t error;
+  // The sweep decided at the last hour of day 0, which no other hour overwrites: the night
+  // launch (hours of day 1..6 together) applies it while its epilogues refill the slots.
+  uint32_t night_sweep, night_vacc_hour;
+  uint64_t night_vacc_thr;
 };
 
 // Launch-table entry: (simulation, first local agent) or (simulation, first hospital cell).

@@ -147,6 +147,35 @@ def test_batch_of_mixed_replicates_matches_single_runs(cfg1):
         assert batch.events(i) == cpu.events()
 
 
+def test_night_launch_in_a_batch_with_sweeps(cfg1):
+    """Day graphs run hours of day 1..6 as one launch when no vaccination falls at 1..5 and
+    no run ends at 1..6: a batch of runs ending at hours of day 0, 7 and 23, with a
+    lockdown (decided at hour of day 0, applied in the night) and vaccinations at hours of
+    day 0 and 6. Each equals its own run."""
+    cfgs, seeds = [], []
+    for r, hours in enumerate((24 * 12 + 1, 24 * 15 + 8, 24 * 20)):
+        c = copy.deepcopy(cfg1)
+        c["hours"] = hours
+        c["disease"]["regular_transmission_rate"] = 0.1 + 0.05 * r
+        c["disease"]["high_transmission_rate"] = 0.4
+        c["starting_infections"]["exposed"] = 5
+        c["interventions"] = [{"Lockdown": {"at_number_of_infections": 30, "essential_workers_population": 0.1}},
+                              {"Vaccinate": {"at_hour": 240, "percent": 0.3}},
+                              {"Vaccinate": {"at_hour": 270, "percent": 0.2}}]
+        cfgs.append(c)
+        seeds.append(2000 + r)
+    say("[parity] night launch: batch of %d" % len(cfgs))
+    batch = G.Batch(cfgs, seeds)
+    rows, stopped = batch.step(500)
+    for i, c in enumerate(cfgs):
+        cpu = O.OracleSim(c, seed=seeds[i])
+        c_rows, c_stop = cpu.step(500)
+        assert np.array_equal(rows[i], c_rows), "run %d rows" % i
+        assert stopped[i] == c_stop
+        assert_agents_equal(batch.agents(i), cpu.agents(), "run %d" % i)
+        assert batch.events(i) == cpu.events()
+
+
 def test_cfg2_1m_first_two_days_bit_exact():
     """1,000,000 agents on 2000x2000: the first 48 hours compared agent by agent."""
     cfg = json.load(open(os.path.join(HERE, "golden", "configs", "cfg2_1m.json")))
