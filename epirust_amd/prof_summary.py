@@ -18,8 +18,9 @@ import json
 import re
 import sys
 
-HOUR_KERNELS = ("k_hospital_scan", "k_relayout", "k_step", "k_commit", "k_quiet_hours")
-QUIET_HOURS = 4  # a k_quiet_hours dispatch runs hours of day 3..6
+HOUR_KERNELS = ("k_hospital_scan", "k_relayout", "k_step", "k_commit", "k_quiet_hours", "k_night")
+# dispatches that close several hours: k_quiet_hours runs hours of day 3..6, k_night 1..6
+SPAN = {"k_commit": 1, "k_quiet_hours": 4, "k_night": 6}
 
 
 def short(name):
@@ -34,9 +35,10 @@ def short(name):
 def by_hour_of_day(trace_csv, hours):
     """Mean duration per kernel for each hour of day over the first `hours` hours of the
     trace (an hour is [k_hospital_scan] [k_relayout] [k_step] k_commit: hours are counted by
-    k_commit dispatches, and a k_quiet_hours dispatch stands for the four hours of day
-    3..6, reported at 3), with the wall time from one hour's first kernel to the next
-    hour's (a quiet dispatch's wall time is split evenly over its four hours)."""
+    k_commit dispatches; a k_quiet_hours dispatch stands for the four hours of day 3..6 and
+    a k_night dispatch for the six of 1..6, reported at their first hour), with the wall
+    time from one hour's first kernel to the next hour's (a dispatch that closes several
+    hours has its wall time split evenly over them)."""
     ks = []
     for r in csv.DictReader(open(trace_csv)):
         k = short(r["Kernel_Name"])
@@ -51,15 +53,13 @@ def by_hour_of_day(trace_csv, hours):
             break
         start.setdefault(hour, s)
         dur[hour % 24][k].append((e - s) / 1e3)
-        if k == "k_commit":
-            span[hour] = 1
-            hour += 1
-        elif k == "k_quiet_hours":
-            span[hour] = QUIET_HOURS
-            for q in range(1, QUIET_HOURS):
+        if k in SPAN:
+            n = SPAN[k]
+            span[hour] = n
+            for q in range(1, n):
                 for kk in HOUR_KERNELS:
                     dur[(hour + q) % 24][kk]  # listed hours, no dispatches
-            hour += QUIET_HOURS
+            hour += n
     wall = collections.defaultdict(list)
     for h in sorted(start):
         n = span.get(h, 1)

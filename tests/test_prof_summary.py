@@ -1,6 +1,6 @@
 """The rocprof summary's grouping of dispatches into simulated hours
 (epirust_amd/prof_summary.py), on a synthetic kernel trace: one hour per k_commit, four
-per k_quiet_hours (hours of day 3..6)."""
+per k_quiet_hours (hours of day 3..6), six per k_night (1..6)."""
 import csv
 
 from epirust_amd import prof_summary as P
@@ -20,8 +20,11 @@ def write_trace(path, kernels):
 
 
 def day(quiet):
-    ks = [("k_commit", 6000), ("k_commit", 9000)]  # hours of day 1, 2
-    ks += [("k_quiet_hours", 8000)] if quiet else [("k_commit", 4000)] * 4
+    if quiet == "night":
+        ks = [("k_night", 11000)]  # hours of day 1..6
+    else:
+        ks = [("k_commit", 6000), ("k_commit", 9000)]  # hours of day 1, 2
+        ks += [("k_quiet_hours", 8000)] if quiet else [("k_commit", 4000)] * 4
     for hd in range(7, 25):
         if hd % 24 == 0:
             ks.append(("k_hospital_scan", 4000))
@@ -41,14 +44,20 @@ def rows(text):
 
 
 def test_hours_are_counted_by_commits_and_quiet_dispatches(tmp_path):
-    for quiet in (False, True):
+    for quiet in (False, True, "night"):
         p = tmp_path / ("q.csv" if quiet else "c.csv")
         write_trace(p, day(quiet) + day(quiet))
         r = rows(P.by_hour_of_day(str(p), 48))
         cols = list(P.HOUR_KERNELS)
-        assert r[1][cols.index("k_commit")] == 6.0 and r[2][cols.index("k_commit")] == 9.0
         assert r[0][cols.index("k_hospital_scan")] == 4.0 and r[0][cols.index("k_step")] == 20.0
         assert r[8][cols.index("k_relayout")] == 5.0 and r[9][cols.index("k_relayout")] == 0.0
+        assert r[7][cols.index("k_step")] == 20.0 and r[7][cols.index("k_commit")] == 10.0
+        if quiet == "night":
+            assert r[1][cols.index("k_night")] == 11.0
+            assert all(sum(r[hd][:len(cols)]) == 0.0 for hd in (2, 3, 4, 5, 6))
+            assert all(r[hd][-1] == 2.0 for hd in (1, 2, 3, 4, 5, 6))  # (11 + 1) / 6
+            continue
+        assert r[1][cols.index("k_commit")] == 6.0 and r[2][cols.index("k_commit")] == 9.0
         if quiet:
             assert r[3][cols.index("k_quiet_hours")] == 8.0
             assert all(sum(r[hd][:len(cols)]) == 0.0 for hd in (4, 5, 6))
