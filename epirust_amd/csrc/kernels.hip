@@ -24,6 +24,8 @@
 //              there k_step resolves the claims of each office's cells among the
 //              block's agents in LDS, and the winners on offices that lie wholly in one
 //              block (all of them when an office fits a block) claim nothing globally.
+//   k_night / k_quiet_hours: in day graphs, hours of day 1..6 (or 3..6) as one launch
+//              when no sweep can be pending inside them (no vaccination there).
 // A sweep decided at hour h (allocation_map.rs:306-337) is applied by the agent pass of
 // hour h+1 (before the routine, in k_step), which is what the reference's next hour sees;
 // sweeps are idempotent, so k_apply_sweep re-applies the last one before the host reads
