@@ -176,6 +176,25 @@ def test_night_launch_in_a_batch_with_sweeps(cfg1):
         assert batch.events(i) == cpu.events()
 
 
+def test_quiet_hours_launch_with_a_vaccination_at_hour_of_day_1(cfg1):
+    """A vaccination at hour of day 1 keeps the night launch off (its sweep is pending at
+    hour of day 2), but hours of day 3..6 still run as one epilogue-only launch in the day
+    graphs. Compared with the oracle over one graph-replayed run."""
+    cfg = interventions_config(cfg1)
+    cfg["hours"] = 24 * 8
+    cfg["interventions"] = [{"Lockdown": {"at_number_of_infections": 20, "essential_workers_population": 0.3}},
+                            {"Vaccinate": {"at_hour": 49, "percent": 0.3}},
+                            {"Vaccinate": {"at_hour": 97, "percent": 0.2}}]
+    say("[parity] quiet hours")
+    gpu = G.Simulation(cfg)
+    cpu = O.OracleSim(cfg)
+    g_rows, g_stop = gpu.step(24 * 7)
+    c_rows, c_stop = cpu.step(24 * 7)
+    assert np.array_equal(g_rows, c_rows) and g_stop == c_stop
+    assert_agents_equal(gpu.agents(), cpu.agents(), "quiet hours")
+    assert gpu.events() == cpu.events()
+
+
 def test_cfg2_1m_first_two_days_bit_exact():
     """1,000,000 agents on 2000x2000: the first 48 hours compared agent by agent."""
     cfg = json.load(open(os.path.join(HERE, "golden", "configs", "cfg2_1m.json")))
