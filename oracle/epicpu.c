@@ -938,6 +938,77 @@ epicpu_sim* epicpu_create(const epicpu_config* cfg, char* err, size_t errlen) {
     return s;
 }
 
+static rect rect_of(const int32_t r[4]) { return (rect){r[0], r[1], r[2], r[3]}; }
+
+epicpu_sim* epicpu_create_scenario(const epicpu_config* cfg, const epicpu_agent_init* agents, uint32_t n,
+                                   uint32_t last_hour, const int32_t* hospital, char* err, size_t errlen) {
+    char msg[256];
+    if (cfg->grid_size == 0 || cfg->grid_size > 32000) { set_err(err, errlen, "grid_size out of supported range"); return NULL; }
+    epicpu_sim* s = (epicpu_sim*)calloc(1, sizeof(epicpu_sim));
+    s->cfg = *cfg;
+    s->cfg.csv_n = 0;
+    s->n = n;
+    s->cfg.number_of_agents = n;
+    s->g = cfg->grid_size;
+    define_geography(s->g, &s->housing, &s->transport, &s->work, &s->hospital);
+    s->houses = make_buildings(s->housing, 2);
+    s->offices = make_buildings(s->work, 10);
+    if (hospital) s->hospital = rect_of(hospital);
+    else if (resize_hospital(&s->hospital, (int32_t)n, 0.002, cfg->hospital_beds_percentage)) {
+        set_err(err, errlen, "attempt to divide by zero");
+        free(s);
+        return NULL;
+    }
+    s->stride = s->g + 1;
+    s->height = s->g + 1;
+    s->thr_regular = epicpu_bernoulli_threshold(cfg->regular_transmission_rate);
+    s->thr_high = epicpu_bernoulli_threshold(cfg->high_transmission_rate);
+    s->thr_death = epicpu_bernoulli_threshold(cfg->death_rate);
+    s->thr_symptoms = epicpu_bernoulli_threshold(1.0 - cfg->percentage_asymptomatic_population);
+    s->thr_severe = epicpu_bernoulli_threshold(cfg->percentage_severe_infected_population);
+    size_t cells = (size_t)s->stride * s->height;
+    s->ag = (citizen*)calloc(n ? n : 1, sizeof(citizen));
+    s->upd = (citizen*)calloc(n ? n : 1, sizeof(citizen));
+    s->pos = (point*)calloc(n ? n : 1, sizeof(point));
+    s->tgt = (point*)calloc(n ? n : 1, sizeof(point));
+    s->current = (uint32_t*)calloc(cells, sizeof(uint32_t));
+    s->upcoming = (uint32_t*)calloc(cells, sizeof(uint32_t));
+    for (uint32_t i = 0; i < n; i++) {
+        const epicpu_agent_init* a = &agents[i];
+        if (a->x < 0 || a->y < 0 || (uint32_t)a->x >= s->stride || (uint32_t)a->y >= s->height ||
+            s->current[cell_index(s, a->x, a->y)] != 0) {
+            snprintf(msg, sizeof msg, "agent %u: cell (%d, %d) is outside the grid or taken", i, a->x, a->y);
+            set_err(err, errlen, msg);
+            epicpu_destroy(s);
+            return NULL;
+        }
+        citizen* c = &s->ag[i];
+        c->id = i;
+        c->home = rect_of(a->home);
+        c->work = rect_of(a->work);
+        c->current_area = rect_of(a->current_area);
+        c->state = (uint8_t)a->state;
+        c->severity = (uint8_t)a->severity;
+        c->work_status = (uint8_t)a->work_status;
+        c->hospitalized = (a->flags & 1) != 0;
+        c->vaccinated = (a->flags & 2) != 0;
+        c->isolated = (a->flags & 4) != 0;
+        c->work_quarantined = (a->flags & 8) != 0;
+        c->uses_pt = (a->flags & 16) != 0;
+        c->infection_day = a->infection_day;
+        c->at_hour = a->at_hour;
+        c->work_start_at = a->work_start_at;
+        c->immunity = a->immunity;
+        s->pos[i] = (point){a->x, a->y};
+        s->current[cell_index(s, a->x, a->y)] = i + 1;
+    }
+    s->counts[0] = last_hour;
+    for (uint32_t i = 0; i < n; i++) update_counts(s->counts, &s->ag[i]);
+    s->hour = last_hour;
+    s->stopped = last_hour + 1 >= cfg->hours;
+    return s;
+}
+
 void epicpu_destroy(epicpu_sim* s) {
     if (!s) return;
     free(s->ag); free(s->upd); free(s->pos); free(s->tgt);

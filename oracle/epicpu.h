@@ -113,6 +113,24 @@ uint32_t epicpu_events(const epicpu_sim* s, uint32_t* hours, uint32_t* kinds, ui
    plus n_houses, n_offices and the transport-point count. */
 void epicpu_geometry(const epicpu_sim* s, int32_t rects[20], uint32_t counts3[3]);
 
+/* Known-answer test hook: one explicit agent. Codes and flag bits are those of
+   epicpu_agents; rects are x0 y0 x1 y1. */
+typedef struct {
+    int32_t x, y;
+    int32_t home[4], work[4], current_area[4];
+    uint32_t state, severity, work_status, flags;
+    uint32_t infection_day, at_hour, work_start_at;
+    int32_t immunity;
+} epicpu_agent_init;
+
+/* A simulation with the config's geography, disease and interventions whose population is
+   exactly `agents` (ids 0..n-1, no synthesis), having run hour `last_hour`: the next
+   epicpu_step simulates last_hour + 1. `hospital` (x0 y0 x1 y1) replaces the resized
+   hospital area when not NULL. Returns NULL on an invalid scenario (a position outside
+   the (g+1)^2 grid, two agents on one cell). */
+epicpu_sim* epicpu_create_scenario(const epicpu_config* cfg, const epicpu_agent_init* agents, uint32_t n,
+                                   uint32_t last_hour, const int32_t* hospital, char* err, size_t errlen);
+
 /* Building blocks exposed for known-answer tests. */
 void epicpu_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void epicpu_define_geography(uint32_t grid_size, int32_t rects[16]);

@@ -80,6 +80,14 @@ class EpicpuAgents(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name, _, _ in AGENT_FIELDS]
 
 
+class EpicpuAgentInit(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int32), ("y", ctypes.c_int32), ("home", ctypes.c_int32 * 4),
+                ("work", ctypes.c_int32 * 4), ("current_area", ctypes.c_int32 * 4), ("state", ctypes.c_uint32),
+                ("severity", ctypes.c_uint32), ("work_status", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("infection_day", ctypes.c_uint32), ("at_hour", ctypes.c_uint32), ("work_start_at", ctypes.c_uint32),
+                ("immunity", ctypes.c_int32)]
+
+
 _lib = None
 
 
@@ -122,6 +130,9 @@ def lib():
         L.epicpu_hospital_counts_updated.argtypes = [ctypes.c_uint32] * 3
         L.epicpu_hospital_should_apply.argtypes = [ctypes.c_int] + [ctypes.c_uint32] * 4
         L.epicpu_first_vacant.argtypes = [_P(ctypes.c_int32), _P(ctypes.c_int32), ctypes.c_uint32, _P(ctypes.c_int32)]
+        L.epicpu_create_scenario.restype = ctypes.c_void_p
+        L.epicpu_create_scenario.argtypes = [_P(EpicpuConfig), ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                             _P(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -220,6 +231,28 @@ class OracleSim:
         if not self.h:
             raise RuntimeError(err.value.decode())
         self.n = self.cfg.number_of_agents
+
+    @classmethod
+    def scenario(cls, cfg, agents, last_hour, hospital=None, seed=None):
+        """A simulation whose population is exactly `agents` (dicts of EpicpuAgentInit
+        fields; ids in list order) that has run hour `last_hour` (known-answer tests)."""
+        self = cls.__new__(cls)
+        self.cfg = config_from_json(cfg, seed)
+        arr = (EpicpuAgentInit * max(len(agents), 1))()
+        for i, a in enumerate(agents):
+            for k, v in a.items():
+                if k in ("home", "work", "current_area"):
+                    getattr(arr[i], k)[:] = list(v)
+                else:
+                    setattr(arr[i], k, v)
+        hosp = _arr(ctypes.c_int32, hospital) if hospital is not None else None
+        err = ctypes.create_string_buffer(512)
+        self.h = lib().epicpu_create_scenario(ctypes.byref(self.cfg), ctypes.cast(arr, ctypes.c_void_p), len(agents),
+                                              last_hour, hosp, err, 512)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        self.n = len(agents)
+        return self
 
     def set_threads(self, threads):
         lib().epicpu_set_threads(self.h, threads)
