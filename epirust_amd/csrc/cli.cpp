@@ -51,9 +51,15 @@ extern "C" int epi_cli_main(int argc, char** argv) {
     usage();
     return 2;
   }
-  for (auto& ch : mode) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
-  if (mode != "standalone") {
+  // clap's ValueEnum names of Mode (main.rs:41-46), matched case-sensitively
+  if (mode == "kafka" || mode == "mpi") {
     std::fprintf(stderr, "error: mode '%s' is not provided by this engine (standalone only)\n", mode.c_str());
+    return 2;
+  }
+  if (mode != "standalone") {
+    std::fprintf(stderr,
+                 "error: invalid value '%s' for '--mode <MODE>'\n  [possible values: kafka, mpi, standalone]\n",
+                 mode.c_str());
     return 2;
   }
   std::printf("Standalone\n");

@@ -15,6 +15,14 @@
  *   - uniform integer on [0,n) = (u32 * n) >> 32;
  *   - Bernoulli(p) = rand 0.8's rule: p == 1 -> true, else u64 < floor(p * 2^64);
  *   - a cell claimed by several agents in one hour goes to the lowest id.
+ *
+ * Built with -DEPICPU_FNV_STORE (librefcpu.so) the same routine runs on the reference's
+ * store instead: citizens held by value in two FNV hash maps keyed by cell, a parallel
+ * map over the current map's buckets, a single-thread merge in bucket order (hash-order
+ * collision winners) and draws from each agent-hour's sequential stream in the
+ * reference's call order. That build is the timed CPU baseline and the stand-in for the
+ * unmodified engine in distributional checks; it is not bit-comparable with
+ * deterministic mode.
  */
 #include "epicpu.h"
 
@@ -227,6 +235,22 @@ typedef struct {
     uint32_t kind;
 } event;
 
+#ifdef EPICPU_FNV_STORE
+/* ------------------------------------------------------------------ refcpu store
+   Compiled only into librefcpu.so, the reference-structure CPU baseline. The store is the
+   reference's CitizenLocationMap (allocation_map.rs:44-65): two FnvHashMap<Point, Citizen>,
+   citizens held by value in the table, swapped and cleared every hour. The table is
+   open-addressed like hashbrown's: one control byte per bucket, a power-of-two bucket
+   count sized for at most 7/8 of the buckets full, home bucket = hash & mask, iteration in
+   bucket order. The hash is FNV-1a 64 (fnv 1.0.6) over the little-endian bytes of x, then
+   y (Point's derived Hash, geography/point.rs:23-28). Probing is linear where hashbrown
+   probes 16-bucket groups triangularly, so the iteration order differs from the Rust
+   engine's; it is a hash order all the same. */
+typedef struct { point key; citizen val; } fnv_entry;
+typedef struct { uint8_t* ctrl; fnv_entry* e; uint64_t mask; } fnv_map;
+typedef struct { point old_cell, new_cell; citizen c; } fnv_update;
+#endif
+
 struct epicpu_sim {
     epicpu_config cfg;
     uint32_t n, g, stride, height;
@@ -251,15 +275,95 @@ struct epicpu_sim {
     uint32_t n_events, cap_events;
     uint32_t threads;
     int ref_order;  /* epicpu_set_reference_order */
+    int random_merge; /* epicpu_set_random_merge */
     uint32_t* perm;
+#ifdef EPICPU_FNV_STORE
+    fnv_map cur, nxt;    /* current_locations, upcoming_locations */
+    fnv_update* updates; /* the parallel map's output, in iteration order */
+#endif
 };
 
+#ifdef EPICPU_FNV_STORE
+static uint64_t fnv_point(point p) {
+    const uint32_t v[2] = {(uint32_t)p.x, (uint32_t)p.y};
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (int k = 0; k < 2; k++)
+        for (int b = 0; b < 4; b++) {
+            h ^= (v[k] >> (8 * b)) & 0xFFu;
+            h *= 0x100000001b3ull;
+        }
+    return h;
+}
+static int fnv_alloc(fnv_map* m, uint32_t n) {
+    uint64_t want = ((uint64_t)n * 8 + 6) / 7, cap = 16;
+    while (cap < want) cap <<= 1;
+    m->mask = cap - 1;
+    m->ctrl = (uint8_t*)calloc(cap, 1);
+    m->e = (fnv_entry*)malloc(cap * sizeof(fnv_entry));
+    return m->ctrl && m->e ? 0 : -1;
+}
+static void fnv_free(fnv_map* m) { free(m->ctrl); free(m->e); m->ctrl = NULL; m->e = NULL; }
+static inline const citizen* fnv_get(const fnv_map* m, point p) {
+    for (uint64_t i = fnv_point(p) & m->mask;; i = (i + 1) & m->mask) {
+        if (!m->ctrl[i]) return NULL;
+        if (m->e[i].key.x == p.x && m->e[i].key.y == p.y) return &m->e[i].val;
+    }
+}
+/* HashMap::entry(p).or_insert(c): the value now stored at p */
+static const citizen* fnv_or_insert(fnv_map* m, point p, const citizen* c) {
+    for (uint64_t i = fnv_point(p) & m->mask;; i = (i + 1) & m->mask) {
+        if (!m->ctrl[i]) {
+            m->ctrl[i] = 1;
+            m->e[i].key = p;
+            m->e[i].val = *c;
+            return &m->e[i].val;
+        }
+        if (m->e[i].key.x == p.x && m->e[i].key.y == p.y) return &m->e[i].val;
+    }
+}
+/* HashMap::insert(p, c): replaces the value at p */
+static void fnv_insert(fnv_map* m, point p, const citizen* c) {
+    for (uint64_t i = fnv_point(p) & m->mask;; i = (i + 1) & m->mask) {
+        if (!m->ctrl[i] || (m->e[i].key.x == p.x && m->e[i].key.y == p.y)) {
+            m->ctrl[i] = 1;
+            m->e[i].key = p;
+            m->e[i].val = *c;
+            return;
+        }
+    }
+}
+/* HashMap::clear: every control byte, O(capacity) */
+static void fnv_clear(fnv_map* m) { memset(m->ctrl, 0, m->mask + 1); }
+#define FOR_EACH_CITIZEN(s, c)                                 \
+    for (uint64_t b_ = 0; b_ <= (s)->cur.mask; b_++)           \
+        if ((s)->cur.ctrl[b_])                                 \
+            for (citizen* c = &(s)->cur.e[b_].val; c; c = NULL)
+#else
+#define FOR_EACH_CITIZEN(s, c) \
+    for (uint32_t i_ = 0; i_ < (s)->n; i_++) for (citizen* c = &(s)->ag[i_]; c; c = NULL)
+#endif
+
 int epicpu_set_reference_order(epicpu_sim* s, int on) {
+#ifdef EPICPU_FNV_STORE
+    return on ? 0 : -2; /* refcpu always draws in the reference's order */
+#endif
     if (on && !s->perm) {
         s->perm = (uint32_t*)malloc((size_t)(s->n ? s->n : 1) * sizeof(uint32_t));
         if (!s->perm) return -1;
     }
     s->ref_order = on ? 1 : 0;
+    return 0;
+}
+
+int epicpu_set_random_merge(epicpu_sim* s, int on) {
+#ifdef EPICPU_FNV_STORE
+    return on ? -2 : 0; /* refcpu merges in bucket order */
+#endif
+    if (on && !s->perm) {
+        s->perm = (uint32_t*)malloc((size_t)(s->n ? s->n : 1) * sizeof(uint32_t));
+        if (!s->perm) return -1;
+    }
+    s->random_merge = on ? 1 : 0;
     return 0;
 }
 
@@ -270,8 +374,21 @@ void epicpu_set_threads(epicpu_sim* s, uint32_t threads) {
 static inline uint32_t cell_index(const epicpu_sim* s, int32_t x, int32_t y) {
     return (uint32_t)y * s->stride + (uint32_t)x;
 }
+/* The snapshot's agent at a cell, or NULL (allocation_map.rs:152-154) */
+static inline const citizen* agent_at(const epicpu_sim* s, point p) {
+#ifdef EPICPU_FNV_STORE
+    return fnv_get(&s->cur, p);
+#else
+    const uint32_t occ = s->current[cell_index(s, p.x, p.y)];
+    return occ ? &s->ag[occ - 1] : NULL;
+#endif
+}
 static inline int is_vacant(const epicpu_sim* s, point p) { /* allocation_map.rs:161-163 */
+#ifdef EPICPU_FNV_STORE
+    return fnv_get(&s->cur, p) == NULL;
+#else
     return s->current[cell_index(s, p.x, p.y)] == 0;
+#endif
 }
 static inline int in_grid(const epicpu_sim* s, int32_t x, int32_t y) { /* allocation_map.rs:156-159 */
     return x >= 0 && y >= 0 && x < (int32_t)s->g && y < (int32_t)s->g;
@@ -398,9 +515,8 @@ static int on_susceptible(const epicpu_sim* s, const citizen* c, point cell, dra
     for (int k = 0; k < 8; k++) {
         int32_t x = cell.x + MOORE_DX[k], y = cell.y + MOORE_DY[k];
         if (!rect_contains(c->current_area, x, y) || !in_grid(s, x, y)) continue;
-        uint32_t occ = s->current[cell_index(s, x, y)];
-        if (!occ) continue;
-        const citizen* nb = &s->ag[occ - 1]; /* snapshot state of the neighbour */
+        const citizen* nb = agent_at(s, (point){x, y}); /* snapshot state of the neighbour */
+        if (!nb) continue;
         if (nb->state != ST_I || nb->hospitalized) continue;
         /* rate(occupant) = get_current_transmission_rate(day + immunity), citizen/mod.rs:182-185 */
         int win = transmission_window(&s->cfg, immunity_day(nb));
@@ -589,8 +705,7 @@ static void process_interventions(epicpu_sim* s) {
         if (cfg->vaccinate_at_hour[k] == h) vi = (int)k;
     if (vi >= 0) {
         uint64_t thr = epicpu_bernoulli_threshold(cfg->vaccinate_percent[vi]);
-        for (uint32_t i = 0; i < s->n; i++) {
-            citizen* c = &s->ag[i];
+        FOR_EACH_CITIZEN(s, c) {
             if (c->state != ST_S) continue;
             uint32_t w[4];
             draw(cfg->seed, c->id, h, SITE_VACCINATE, 0, w);
@@ -600,13 +715,13 @@ static void process_interventions(epicpu_sim* s) {
     }
     if (cfg->has_lockdown && epicpu_lockdown_should_apply(s->locked, cfg->lockdown_at_number_of_infections, h, s->counts[3])) {
         s->locked = 1;
-        for (uint32_t i = 0; i < s->n; i++)
-            if (s->ag[i].work_status != WS_ESSENTIAL) s->ag[i].isolated = 1;
+        FOR_EACH_CITIZEN(s, c)
+            if (c->work_status != WS_ESSENTIAL) c->isolated = 1;
         push_event(s, h, 1);
     }
     if (epicpu_lockdown_should_unlock(s->locked, s->zero_infection_hour, h)) {
-        for (uint32_t i = 0; i < s->n; i++)
-            if (s->ag[i].isolated) s->ag[i].isolated = 0;
+        FOR_EACH_CITIZEN(s, c)
+            if (c->isolated) c->isolated = 0;
         s->locked = 0;
         s->zero_infection_hour = 0;
         push_event(s, h, 2);
@@ -631,6 +746,93 @@ static void update_counts(uint32_t counts[7], const citizen* c) { /* counts.rs:1
     default: counts[6]++; break;
     }
 }
+
+#ifdef EPICPU_FNV_STORE
+/* refcpu's hour (allocation_map.rs:82-134): rayon's par_iter over the current map splits
+   its buckets into contiguous ranges and `collect` keeps iteration order, so thread k
+   counts the agents of its bucket range, then writes their updates at its offset. */
+typedef struct {
+    epicpu_sim* s;
+    uint32_t h;
+    uint64_t begin, end;
+    uint32_t count, offset;
+} fnv_job;
+
+static void* fnv_count_range(void* arg) {
+    fnv_job* j = (fnv_job*)arg;
+    uint32_t n = 0;
+    for (uint64_t b = j->begin; b < j->end; b++) n += j->s->cur.ctrl[b];
+    j->count = n;
+    return NULL;
+}
+
+static void* fnv_map_range(void* arg) {
+    fnv_job* j = (fnv_job*)arg;
+    fnv_update* u = j->s->updates + j->offset;
+    for (uint64_t b = j->begin; b < j->end; b++) {
+        if (!j->s->cur.ctrl[b]) continue;
+        const fnv_entry* e = &j->s->cur.e[b];
+        u->old_cell = e->key;
+        u->c = e->val;
+        u->new_cell = routine(j->s, &u->c, e->key, j->h);
+        u++;
+    }
+    return NULL;
+}
+
+static void fnv_run(fnv_job* jobs, uint32_t nt, void* (*fn)(void*)) {
+    pthread_t th[64];
+    for (uint32_t k = 1; k < nt; k++) pthread_create(&th[k], NULL, fn, &jobs[k]);
+    fn(&jobs[0]);
+    for (uint32_t k = 1; k < nt; k++) pthread_join(th[k], NULL);
+}
+
+static int simulate(epicpu_sim* s, uint32_t h) {
+    for (int k = 1; k < 7; k++) s->counts[k] = 0;
+    const uint64_t cap = s->cur.mask + 1;
+    uint32_t nt = s->threads > 1 && s->n >= 4096 ? s->threads : 1;
+    fnv_job jobs[64];
+    const uint64_t chunk = (cap + nt - 1) / nt;
+    for (uint32_t k = 0; k < nt; k++) {
+        const uint64_t b = k * chunk < cap ? k * chunk : cap, e = b + chunk < cap ? b + chunk : cap;
+        jobs[k] = (fnv_job){s, h, b, e, 0, 0};
+    }
+    fnv_run(jobs, nt, fnv_count_range);
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < nt; k++) { jobs[k].offset = off; off += jobs[k].count; }
+    if (off != s->n) return -2;
+    fnv_run(jobs, nt, fnv_map_range);
+    /* the sequential merge, in iteration order (allocation_map.rs:93-125) */
+    for (uint32_t k = 0; k < s->n; k++) {
+        const fnv_update* u = &s->updates[k];
+        const citizen* at_new = fnv_or_insert(&s->nxt, u->new_cell, &u->c);
+        if (at_new->id != u->c.id) fnv_insert(&s->nxt, u->old_cell, &u->c);
+        update_counts(s->counts, &u->c);
+    }
+    /* swap (allocation_map.rs:131-134): the old current map is cleared and becomes upcoming */
+    fnv_clear(&s->cur);
+    fnv_map t = s->cur; s->cur = s->nxt; s->nxt = t;
+    uint32_t total = 0;
+    for (int k = 1; k < 7; k++) total += s->counts[k];
+    return total == s->n ? 0 : -2;
+}
+
+/* CitizenLocationMap::new (allocation_map.rs:52-65): agents inserted in creation order.
+   The dense arrays that population synthesis filled are released. */
+static int fnv_build(epicpu_sim* s) {
+    if (fnv_alloc(&s->cur, s->n) || fnv_alloc(&s->nxt, s->n)) return -1;
+    s->updates = (fnv_update*)malloc((size_t)(s->n ? s->n : 1) * sizeof(fnv_update));
+    if (!s->updates) return -1;
+    for (uint32_t i = 0; i < s->n; i++) fnv_insert(&s->cur, s->pos[i], &s->ag[i]);
+    free(s->ag); free(s->upd); free(s->pos); free(s->tgt); free(s->current); free(s->upcoming);
+    s->ag = s->upd = NULL;
+    s->pos = s->tgt = NULL;
+    s->current = s->upcoming = NULL;
+    s->ref_order = 1; /* draws from each agent-hour's sequential stream, in the reference's call order */
+    return 0;
+}
+#else
+static int fnv_build(epicpu_sim* s) { (void)s; return 0; }
 
 /* The parallel map of allocation_map.rs:82-92: every agent's routine reads only the
    start-of-hour snapshot, so agents can be split over threads like rayon's par_iter. */
@@ -665,7 +867,7 @@ static int simulate(epicpu_sim* s, uint32_t h) {
     /* Sequential merge. Iterating in id order makes "first inserted wins" equal to
        "lowest agent id wins". Targets are vacant in the snapshot or the agent's own
        cell, so a loser's old cell is always free in upcoming. */
-    if (s->ref_order) { /* merge in a uniformly random order (stands in for hash order) */
+    if (s->ref_order || s->random_merge) { /* a uniformly random merge order (stands in for hash order) */
         draws d;
         memset(&d, 0, sizeof d);
         d.s = s; d.id = 0; d.hour = h; d.site = SITE_REF_MERGE; d.spos = 4;
@@ -677,7 +879,7 @@ static int simulate(epicpu_sim* s, uint32_t h) {
         }
     }
     for (uint32_t k = 0; k < s->n; k++) {
-        const uint32_t i = s->ref_order ? s->perm[k] : k;
+        const uint32_t i = s->ref_order || s->random_merge ? s->perm[k] : k;
         point oc = s->pos[i], nc = s->tgt[i];
         uint32_t ci = cell_index(s, nc.x, nc.y);
         if (s->upcoming[ci] == 0) {
@@ -698,6 +900,8 @@ static int simulate(epicpu_sim* s, uint32_t h) {
     for (int k = 1; k < 7; k++) total += s->counts[k];
     return total == s->n ? 0 : -2;
 }
+
+#endif
 
 int epicpu_step(epicpu_sim* s, uint32_t n_hours, uint32_t* rows, uint32_t* n_written, int* stopped) {
     uint32_t written = 0;
@@ -935,6 +1139,11 @@ epicpu_sim* epicpu_create(const epicpu_config* cfg, char* err, size_t errlen) {
     s->counts[3] = total_si - cfg->si_exposed;
     s->hour = 0;
     s->stopped = cfg->hours <= 1;
+    if (fnv_build(s)) {
+        set_err(err, errlen, "out of memory");
+        epicpu_destroy(s);
+        return NULL;
+    }
     return s;
 }
 
@@ -1006,6 +1215,11 @@ epicpu_sim* epicpu_create_scenario(const epicpu_config* cfg, const epicpu_agent_
     for (uint32_t i = 0; i < n; i++) update_counts(s->counts, &s->ag[i]);
     s->hour = last_hour;
     s->stopped = last_hour + 1 >= cfg->hours;
+    if (fnv_build(s)) {
+        set_err(err, errlen, "out of memory");
+        epicpu_destroy(s);
+        return NULL;
+    }
     return s;
 }
 
@@ -1013,6 +1227,9 @@ void epicpu_destroy(epicpu_sim* s) {
     if (!s) return;
     free(s->ag); free(s->upd); free(s->pos); free(s->tgt);
     free(s->current); free(s->upcoming); free(s->events); free(s->perm);
+#ifdef EPICPU_FNV_STORE
+    fnv_free(&s->cur); fnv_free(&s->nxt); free(s->updates);
+#endif
     free(s);
 }
 
@@ -1033,10 +1250,19 @@ void epicpu_geometry(const epicpu_sim* s, int32_t r[20], uint32_t c3[3]) {
 }
 
 void epicpu_read_agents(const epicpu_sim* s, epicpu_agents* o) {
+#ifdef EPICPU_FNV_STORE
+    for (uint64_t b = 0; b <= s->cur.mask; b++) {
+        if (!s->cur.ctrl[b]) continue;
+        const citizen* c = &s->cur.e[b].val;
+        const uint32_t i = c->id;
+        o->x[i] = s->cur.e[b].key.x;
+        o->y[i] = s->cur.e[b].key.y;
+#else
     for (uint32_t i = 0; i < s->n; i++) {
         const citizen* c = &s->ag[i];
         o->x[i] = s->pos[i].x;
         o->y[i] = s->pos[i].y;
+#endif
         o->state[i] = c->state;
         o->severity[i] = c->state == ST_I ? c->severity : 0;
         o->infection_day[i] = c->state == ST_I ? c->infection_day : 0;

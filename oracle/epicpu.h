@@ -100,6 +100,10 @@ void epicpu_destroy(epicpu_sim* s);
    gen_bool per infectious neighbour) and the merge visits agents in a random order per
    hour (standing in for hash order) instead of id order. Not bit-comparable with the GPU. */
 int epicpu_set_reference_order(epicpu_sim* s, int on);
+/* Deterministic-mode draws with the merge of reference-order mode: collisions go to the
+   agent visited first in a random order per hour instead of the lowest id. For
+   distributional checks of the collision rule alone; not bit-comparable with the GPU. */
+int epicpu_set_random_merge(epicpu_sim* s, int on);
 /* Threads for the per-agent map (the merge stays sequential); results do not depend on it. */
 void epicpu_set_threads(epicpu_sim* s, uint32_t threads);
 /* Runs up to n_hours hours; rows gets 7 u32 per hour (hour,S,E,I,H,R,D). */

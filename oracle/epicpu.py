@@ -18,6 +18,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libepicpu.so")
+REF_LIB_PATH = os.path.join(HERE, "librefcpu.so")
 MAX_VACC = 4096
 
 
@@ -88,20 +89,20 @@ class EpicpuAgentInit(ctypes.Structure):
                 ("immunity", ctypes.c_int32)]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(path=LIB_PATH):
+    if path not in _libs:
+        if not os.path.exists(path):
             raise RuntimeError("oracle library missing: build it with `make -C oracle`")
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         L.epicpu_create.restype = ctypes.c_void_p
         L.epicpu_create.argtypes = [_P(EpicpuConfig), ctypes.c_char_p, ctypes.c_size_t]
         L.epicpu_destroy.argtypes = [ctypes.c_void_p]
         L.epicpu_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
         L.epicpu_set_reference_order.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.epicpu_set_random_merge.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.epicpu_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_uint32), _P(ctypes.c_uint32),
                                   _P(ctypes.c_int)]
         L.epicpu_read_agents.argtypes = [ctypes.c_void_p, _P(EpicpuAgents)]
@@ -133,8 +134,8 @@ def lib():
         L.epicpu_create_scenario.restype = ctypes.c_void_p
         L.epicpu_create_scenario.argtypes = [_P(EpicpuConfig), ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                              _P(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _arr(ctype, values):
@@ -224,10 +225,13 @@ def alloc_agents(n):
 class OracleSim:
     """One deterministic-mode simulation on the CPU oracle."""
 
+    LIB = LIB_PATH
+
     def __init__(self, cfg, seed=None):
+        self.L = lib(self.LIB)
         self.cfg = config_from_json(cfg, seed)
         err = ctypes.create_string_buffer(512)
-        self.h = lib().epicpu_create(ctypes.byref(self.cfg), err, 512)
+        self.h = self.L.epicpu_create(ctypes.byref(self.cfg), err, 512)
         if not self.h:
             raise RuntimeError(err.value.decode())
         self.n = self.cfg.number_of_agents
@@ -237,6 +241,7 @@ class OracleSim:
         """A simulation whose population is exactly `agents` (dicts of EpicpuAgentInit
         fields; ids in list order) that has run hour `last_hour` (known-answer tests)."""
         self = cls.__new__(cls)
+        self.L = lib(cls.LIB)
         self.cfg = config_from_json(cfg, seed)
         arr = (EpicpuAgentInit * max(len(agents), 1))()
         for i, a in enumerate(agents):
@@ -247,7 +252,7 @@ class OracleSim:
                     setattr(arr[i], k, v)
         hosp = _arr(ctypes.c_int32, hospital) if hospital is not None else None
         err = ctypes.create_string_buffer(512)
-        self.h = lib().epicpu_create_scenario(ctypes.byref(self.cfg), ctypes.cast(arr, ctypes.c_void_p), len(agents),
+        self.h = self.L.epicpu_create_scenario(ctypes.byref(self.cfg), ctypes.cast(arr, ctypes.c_void_p), len(agents),
                                               last_hour, hosp, err, 512)
         if not self.h:
             raise RuntimeError(err.value.decode())
@@ -255,17 +260,23 @@ class OracleSim:
         return self
 
     def set_threads(self, threads):
-        lib().epicpu_set_threads(self.h, threads)
+        self.L.epicpu_set_threads(self.h, threads)
 
     def set_reference_order(self, on=True):
         """Sequential per-agent-hour streams in the reference's call order and a random
         merge order (distributional checks; not bit-comparable with deterministic mode)."""
-        if lib().epicpu_set_reference_order(self.h, 1 if on else 0) != 0:
+        if self.L.epicpu_set_reference_order(self.h, 1 if on else 0) != 0:
             raise MemoryError("epicpu_set_reference_order")
+
+    def set_random_merge(self, on=True):
+        """Deterministic-mode draws, collisions won by the first agent of a random order per
+        hour instead of the lowest id (the collision rule alone, for distributional checks)."""
+        if self.L.epicpu_set_random_merge(self.h, 1 if on else 0) != 0:
+            raise MemoryError("epicpu_set_random_merge")
 
     def close(self):
         if self.h:
-            lib().epicpu_destroy(self.h)
+            self.L.epicpu_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -275,7 +286,7 @@ class OracleSim:
         rows = np.zeros((max(n_hours, 1), 7), dtype=np.uint32)
         nw = ctypes.c_uint32(0)
         st = ctypes.c_int(0)
-        rc = lib().epicpu_step(self.h, n_hours, rows.ctypes.data_as(_P(ctypes.c_uint32)), ctypes.byref(nw),
+        rc = self.L.epicpu_step(self.h, n_hours, rows.ctypes.data_as(_P(ctypes.c_uint32)), ctypes.byref(nw),
                                ctypes.byref(st))
         if rc != 0:
             raise RuntimeError("epicpu_step failed: %d" % rc)
@@ -283,28 +294,47 @@ class OracleSim:
 
     def agents(self):
         arrays, s = alloc_agents(self.n)
-        lib().epicpu_read_agents(self.h, ctypes.byref(s))
+        self.L.epicpu_read_agents(self.h, ctypes.byref(s))
         arrays["current_area"] = arrays["current_area"].reshape(-1, 4)
         return arrays
 
     def counts(self):
         out = (ctypes.c_uint32 * 7)()
-        lib().epicpu_counts(self.h, out)
+        self.L.epicpu_counts(self.h, out)
         return list(out)
 
     def events(self):
         cap = 65536
         hours = (ctypes.c_uint32 * cap)()
         kinds = (ctypes.c_uint32 * cap)()
-        n = lib().epicpu_events(self.h, hours, kinds, cap)
+        n = self.L.epicpu_events(self.h, hours, kinds, cap)
         return [(hours[i], kinds[i]) for i in range(min(n, cap))]
 
     def geometry(self):
         r = (ctypes.c_int32 * 20)()
         c = (ctypes.c_uint32 * 3)()
-        lib().epicpu_geometry(self.h, r, c)
+        self.L.epicpu_geometry(self.h, r, c)
         rects = [tuple(r[4 * i:4 * i + 4]) for i in range(5)]
         return rects, list(c)
+
+
+class RefCpuSim(OracleSim):
+    """The same routine on the reference's store (oracle/librefcpu.so): citizens by value
+    in two FNV hash maps keyed by cell, swapped every hour (allocation_map.rs:44-65), a
+    threaded map over the buckets, a single-thread merge in bucket order so collision
+    winners follow hash order, and draws in the reference's call order. The timed CPU
+    baseline and the stand-in for the unmodified engine; not bit-comparable with
+    deterministic mode."""
+
+    LIB = REF_LIB_PATH
+
+    def set_reference_order(self, on=True):
+        if not on:
+            raise ValueError("refcpu always draws in the reference's order")
+
+    def set_random_merge(self, on=True):
+        if on:
+            raise ValueError("refcpu merges in bucket order")
 
 
 def philox(ctr, key):

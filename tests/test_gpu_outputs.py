@@ -1,0 +1,193 @@
+"""The drop-in output files, byte for byte.
+
+What the reference writes for a standalone run (engine-app -c CFG -o DIR):
+  * DIR/output/ is created when absent, errors ignored (engine/src/utils/util.rs:28-35);
+  * the file stem is simulation_<engine id>_<UTC [year]-[month]-[day]T[hour]:[minute]:[second]>
+    (util.rs:36-43), engine id "0" in standalone mode (engine_app.rs:30);
+  * the counts CSV is <stem>.csv (epidemiology_simulation.rs:144-147), written by the csv
+    crate from serde's field order of Counts (models/events/counts.rs:26-34): a header
+    row, then one row per simulated hour, comma separated, '\\n' terminated, no quoting of
+    integers (listeners/csv_service.rs:44-71);
+  * the interventions report is <stem>_interventions.json (epidemiology_simulation.rs:150):
+    serde_json::to_writer of [{"hour","intervention","data"}] with data parsed from the
+    intervention's json_data (listeners/intervention_reporter.rs:28-63; names and data in
+    interventions/lockdown.rs:103-115, hospital.rs:77-85, vaccination.rs:57-65), compact,
+    no trailing newline;
+  * both listener paths are pushed onto EPI_OUTPUT_DIR (default ".")
+    (utils/environment.rs:33-38; csv_service.rs:64-66; intervention_reporter.rs:54-56),
+    so a relative DIR lands under EPI_OUTPUT_DIR while the output/ directory itself is
+    created relative to the working directory.
+
+The expected bytes are built here from the oracle's rows and events with those rules.
+"""
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+import epirust_amd as G
+from oracle import epicpu as O
+
+from gpu_helpers import interventions_config, say
+
+pytestmark = pytest.mark.gpu
+
+STEM = re.compile(r"^simulation_0_(\d{4})-(\d{2})-(\d{2})T(\d{2}):(\d{2}):(\d{2})$")
+NAMES = {0: ("vaccination", {}), 1: ("lockdown", {"status": "locked_down"}),
+         2: ("lockdown", {"status": "lockdown_revoked"}), 3: ("build_new_hospital", {})}
+
+
+def expected_csv(rows):
+    out = "hour,susceptible,exposed,infected,hospitalized,recovered,deceased\n"
+    return out + "".join(",".join(str(int(v)) for v in r) + "\n" for r in rows)
+
+
+def expected_json(events):
+    rep = [{"hour": int(h), "intervention": NAMES[k][0], "data": NAMES[k][1]} for h, k in events]
+    return json.dumps(rep, separators=(",", ":"))
+
+
+def oracle_outputs(cfg, hours):
+    cpu = O.OracleSim(cfg)
+    rows, _ = cpu.step(hours)
+    ev = cpu.events()
+    cpu.close()
+    return expected_csv(rows), expected_json(ev), len(rows), ev
+
+
+def run_config(cfg1):
+    cfg = interventions_config(cfg1)
+    cfg["hours"] = 700
+    return cfg
+
+
+def split_outputs(csv_path):
+    d, name = os.path.split(csv_path)
+    assert name.endswith(".csv")
+    stem = name[:-4]
+    m = STEM.match(stem)
+    assert m, name
+    return d, stem
+
+
+def test_epi_run_files_match_oracle_bytes(cfg1, tmp_path, monkeypatch):
+    cfg = run_config(cfg1)
+    csv_text, json_text, n_rows, ev = oracle_outputs(cfg, cfg["hours"])
+    kinds = {k for _, k in ev}
+    assert {0, 1, 3} <= kinds, ev  # vaccination, lockdown and hospital growth all report
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("EPI_OUTPUT_DIR", raising=False)
+    say("[outputs] epi_run, %d agents, %d hours" % (cfg["population"]["Auto"]["number_of_agents"], cfg["hours"]))
+    sim = G.Simulation(cfg)
+    path = sim.run(str(tmp_path / "run"))
+    sim.close()
+    d, stem = split_outputs(path)
+    assert os.path.realpath(d) == os.path.realpath(str(tmp_path / "run" / "output"))
+    with open(path, "rb") as f:
+        got_csv = f.read()
+    with open(os.path.join(d, stem + "_interventions.json"), "rb") as f:
+        got_json = f.read()
+    assert got_csv == csv_text.encode()
+    assert got_csv.count(b"\n") == n_rows + 1
+    assert got_json == json_text.encode()
+    assert sorted(os.listdir(d)) == sorted([stem + ".csv", stem + "_interventions.json"])
+
+
+def test_epi_run_without_interventions_writes_empty_report(cfg1, tmp_path, monkeypatch):
+    cfg = dict(cfg1)
+    cfg["population"] = {"Auto": dict(cfg1["population"]["Auto"], number_of_agents=1500)}
+    cfg["geography_parameters"] = dict(cfg1["geography_parameters"], grid_size=80)
+    cfg["hours"] = 300
+    csv_text, json_text, _, ev = oracle_outputs(cfg, cfg["hours"])
+    assert ev == [] and json_text == "[]"
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("EPI_OUTPUT_DIR", raising=False)
+    path = G.EngineApp.start_standalone(cfg, "out")
+    d, stem = split_outputs(path)
+    assert d == "./out/output"  # EPI_OUTPUT_DIR defaults to "."
+    assert open(path, "rb").read() == csv_text.encode()
+    assert open(os.path.join(d, stem + "_interventions.json"), "rb").read() == b"[]"
+
+
+def test_output_dir_env_prefixes_relative_paths(cfg1, tmp_path, monkeypatch):
+    """The listeners write under EPI_OUTPUT_DIR; the output/ directory is created relative
+    to the working directory (util.rs:28-35), so here both are made to coincide."""
+    cfg = run_config(cfg1)
+    cfg["hours"] = 200
+    csv_text, json_text, _, _ = oracle_outputs(cfg, cfg["hours"])
+    work = tmp_path / "cwd"
+    prefix = tmp_path / "prefix"
+    (prefix / "rel" / "output").mkdir(parents=True)
+    work.mkdir()
+    monkeypatch.chdir(work)
+    monkeypatch.setenv("EPI_OUTPUT_DIR", str(prefix))
+    sim = G.Simulation(cfg)
+    path = sim.run("rel")
+    sim.close()
+    d, stem = split_outputs(path)
+    assert os.path.realpath(d) == os.path.realpath(str(prefix / "rel" / "output"))
+    assert (work / "rel" / "output").is_dir()  # created relative to the working directory
+    assert os.listdir(str(work / "rel" / "output")) == []
+    assert open(path, "rb").read() == csv_text.encode()
+    assert open(os.path.join(d, stem + "_interventions.json"), "rb").read() == json_text.encode()
+
+
+def test_absolute_output_dir_ignores_env(cfg1, tmp_path, monkeypatch):
+    cfg = run_config(cfg1)
+    cfg["hours"] = 120
+    csv_text, _, _, _ = oracle_outputs(cfg, cfg["hours"])
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("EPI_OUTPUT_DIR", str(tmp_path / "elsewhere"))
+    sim = G.Simulation(cfg)
+    path = sim.run(str(tmp_path / "abs"))
+    sim.close()
+    d, _ = split_outputs(path)
+    assert os.path.realpath(d) == os.path.realpath(str(tmp_path / "abs" / "output"))
+    assert open(path, "rb").read() == csv_text.encode()
+    assert not (tmp_path / "elsewhere").exists()
+
+
+def test_cli_executable_end_to_end(cfg1, tmp_path):
+    """epirust-gpu -c CFG -o DIR (engine-app/src/main.rs:60-87, 167-173) writes the same
+    files as the reference's standalone run; the config's seed key selects the streams."""
+    cfg = run_config(cfg1)
+    cfg["seed"] = 99
+    csv_text, json_text, n_rows, _ = oracle_outputs(cfg, cfg["hours"])
+    cfg_path = tmp_path / "cfg.json"
+    cfg_path.write_text(json.dumps(cfg))
+    env = dict(os.environ)
+    env.pop("EPI_OUTPUT_DIR", None)
+    say("[outputs] epirust-gpu executable")
+    p = subprocess.run([G.EXE_PATH, "-c", str(cfg_path), "-m", "standalone", "-t", "4", "-o", str(tmp_path / "cli")],
+                       cwd=str(tmp_path), env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr
+    lines = p.stdout.splitlines()
+    assert lines[0] == "Standalone"
+    assert any(l.startswith("Number of iterations: %d," % n_rows) for l in lines), lines
+    out_dir = tmp_path / "cli" / "output"
+    files = sorted(os.listdir(str(out_dir)))
+    assert len(files) == 2 and files[0].endswith(".csv") and files[1].endswith("_interventions.json")
+    stem = files[0][:-4]
+    assert STEM.match(stem) and files[1] == stem + "_interventions.json"
+    assert (out_dir / files[0]).read_bytes() == csv_text.encode()
+    assert (out_dir / files[1]).read_bytes() == json_text.encode()
+
+
+def test_cli_entry_point_matches_executable(cfg1, tmp_path, monkeypatch):
+    """epi_cli_main through the C ABI (what a Rust main() would call)."""
+    cfg = run_config(cfg1)
+    cfg["hours"] = 150
+    cfg["seed"] = 5
+    csv_text, _, _, _ = oracle_outputs(cfg, cfg["hours"])
+    cfg_path = tmp_path / "c.json"
+    cfg_path.write_text(json.dumps(cfg))
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("EPI_OUTPUT_DIR", raising=False)
+    assert G.cli(["--config", str(cfg_path), "--output-dir=" + str(tmp_path / "abi")]) == 0
+    out_dir = tmp_path / "abi" / "output"
+    csvs = [f for f in os.listdir(str(out_dir)) if f.endswith(".csv")]
+    assert len(csvs) == 1
+    assert (out_dir / csvs[0]).read_bytes() == csv_text.encode()
