@@ -61,34 +61,45 @@ def cpu_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
 
-def oracle_rate(cfgs, target_seconds, seeds=None, threads=None):
-    """CPU oracle (C restatement of the reference step: per-agent map over the
-    start-of-hour snapshot on `threads` threads, sequential merge) on the first hours of
-    the given configs, bounded to about target_seconds. Returns (agent-hours/s, text)."""
+def refcpu_rate(cfgs, target_seconds, threads=None, samples=1, window_hours=0):
+    """refcpu (oracle/librefcpu.so: the reference's step structure restated in C: citizens
+    held by value in two FNV hash maps keyed by cell, a `threads`-thread map over the
+    current map's buckets, a single-thread merge in bucket order, swap and clear,
+    allocation_map.rs:44-134) on the first hours of the given configs: exactly
+    window_hours hours if given, else whole hours until about target_seconds have passed.
+    Population synthesis is outside the timed region, as the reference's timer starts
+    after Epidemiology::new. With several samples (fresh simulations over the same hours)
+    the median rate is reported. Returns (agent-hours/s, text)."""
     from oracle import epicpu as O
     threads = threads or cpu_threads()
-    agent_hours = 0
-    spent = 0.0
-    hours_by = []
-    for i, cfg in enumerate(cfgs):
-        sim = O.OracleSim(cfg, seed=None if seeds is None else seeds[i])
-        sim.set_threads(threads)
-        n = cfg["population"]["Auto"]["number_of_agents"]
-        budget = target_seconds / len(cfgs)
-        t0 = time.perf_counter()
-        h = 0
-        while time.perf_counter() - t0 < budget and h < cfg["hours"] - 1:
-            rows, stopped = sim.step(4)
-            h += len(rows)
-            if stopped:
-                break
-        dt = time.perf_counter() - t0
-        sim.close()
-        agent_hours += n * h
-        spent += dt
-        hours_by.append(h)
-    return agent_hours / spent, "oracle/epicpu.c, %d threads, first %s hours of %d config(s), %.1f s" % (
-        threads, "/".join(str(h) for h in hours_by[:4]) + ("/..." if len(hours_by) > 4 else ""), len(cfgs), spent)
+    rates, hours_by, spent_all = [], [], 0.0
+    for _ in range(samples):
+        agent_hours, spent, hours_by = 0, 0.0, []
+        for cfg in cfgs:
+            sim = O.RefCpuSim(cfg)
+            sim.set_threads(threads)
+            n = cfg["population"]["Auto"]["number_of_agents"]
+            budget = target_seconds / len(cfgs)
+            t0 = time.perf_counter()
+            h = 0
+            while (h < window_hours if window_hours else time.perf_counter() - t0 < budget) and h < cfg["hours"] - 1:
+                rows, stopped = sim.step(window_hours - h if window_hours else 1)
+                h += len(rows)
+                if stopped:
+                    break
+            dt = time.perf_counter() - t0
+            sim.close()
+            agent_hours += n * h
+            spent += dt
+            hours_by.append(h)
+        rates.append(agent_hours / spent)
+        spent_all += spent
+    rates.sort()
+    return rates[len(rates) // 2], (
+        "refcpu (oracle/librefcpu.so, the reference's hash-map step structure restated in C), %d threads, "
+        "first %s hours of %d config(s), median of %d samples (%s agent-h/s), %.1f s in all" % (
+            threads, "/".join(str(h) for h in hours_by[:4]) + ("/..." if len(hours_by) > 4 else ""), len(cfgs),
+            samples, ", ".join("%.3g" % r for r in rates), spent_all))
 
 
 _hip = None
@@ -175,7 +186,7 @@ def bench_replicates(G, cpu_seconds):
            "total_agents": sum(c["population"]["Auto"]["number_of_agents"] for c in cfgs),
            "block": block or 256, "hours_per_replicate": 2399}
     if cpu_seconds > 0:
-        rate, sample = oracle_rate(cfgs[:4], cpu_seconds)
+        rate, sample = refcpu_rate(cfgs[:4], cpu_seconds)
         full = sum(c["population"]["Auto"]["number_of_agents"] * 2399 for c in cfgs)
         out["cpu_baseline"] = {"value": len(cfgs) / (full / rate), "unit": "replicates/s", "cores": cpu_threads(),
                                "kind": "port", "sample": sample + "; replicates run serially, projected "
@@ -210,7 +221,7 @@ def bench_interventions(G, cpu_seconds):
                       "build_new_hospital": sum(1 for e in ev if e[1] == 3)},
            "peak_infected": int(rows[:, 3].max()) if len(rows) else 0}
     if cpu_seconds > 0:
-        rate, sample = oracle_rate([cfg], cpu_seconds)
+        rate, sample = refcpu_rate([cfg], cpu_seconds)
         out["cpu_baseline"] = {"value": rate, "unit": "agent-hours/s", "cores": cpu_threads(), "kind": "port",
                                "sample": sample}
     return out
@@ -236,6 +247,35 @@ def bench_init(G):
     return out
 
 
+def bench_surrogate(n_train, n_eval, seeds, epochs):
+    """The learned surrogate on a reduced, stated sweep: n_train training simulations
+    (100,000 agents, 632x632, 2,400 hours, Sobol points over the ranges of
+    tests/golden/configs/cfg5_surrogate.json), two networks trained with the bf16-MFMA
+    kernels, then n_eval held-out points answered with the 5%-error simulator fallback and
+    timed against the plain sweep of `seeds`-seed simulation ensembles, which also judges
+    the error (peak infected and final size). The full sweep (20,000 / 500 / 32) is
+    `python -m epirust_amd.surrogate`."""
+    import tempfile
+    from epirust_amd import surrogate
+    with tempfile.TemporaryDirectory() as d:
+        m = surrogate.run_pipeline(d, n_train=n_train, n_eval=n_eval, eval_seeds=seeds, epochs=epochs,
+                                   log=lambda line: print(line, file=sys.stderr, flush=True))
+    cal = m["calibrated"]
+    return {"metric": "surrogate speedup @<=5% err", "value": m["speedup_vs_plain_sweep"], "unit": "x",
+            "surrogate_points_per_s": m["surrogate_points_per_s"], "plain_points_per_s": m["plain_points_per_s"],
+            "fraction_within_5pct": m["with_fallback_fraction_within"], "fallback_fraction": m["fallback_fraction"],
+            "surrogate_alone_fraction_within_5pct": m["surrogate_fraction_within"],
+            "nearest_neighbour_fraction_within_5pct": m["nn_fraction_within"],
+            "calibrated_gate": {"speedup": cal["speedup_vs_plain_sweep"],
+                                "fraction_within_5pct": cal["test_fraction_within"],
+                                "fallback_fraction": cal["test_fallback_fraction"]},
+            "config": {"train_simulations": n_train, "eval_points": n_eval, "seeds_per_point": seeds,
+                       "epochs": epochs, "agents": m["agents"], "grid_size": m["grid_size"], "hours": m["hours"],
+                       "note": "reduced sweep; the full one is 20,000 training simulations and 500 points"},
+            "seconds": {"train_simulations": m["train_sim_seconds"], "training": m["train_seconds"],
+                        "plain_sweep": m["plain_sweep_seconds"], "surrogate_sweep": m["surrogate_sweep_seconds"]}}
+
+
 def profile_driver(args, G):
     import ctypes
     with open(args.config) as f:
@@ -257,6 +297,26 @@ def profile_driver(args, G):
     print(json.dumps({"profile_hours": prof["hours"], "kernel_ms": prof, "calibration_copy_bytes": nbytes}))
 
 
+def profile_gemm(shape, reps):
+    """Profiling driver for the surrogate's MFMA GEMM (k_gemm): `reps` calls of
+    epi_gemm_bf16 at M x N x K with random bf16 operands (host buffers; rocprofv3's kernel
+    trace gives the kernel's own duration)."""
+    import ctypes
+    import numpy as np
+    import epirust_amd as G
+    m, n, k = shape
+    rng = np.random.default_rng(0)
+    a = (rng.standard_normal((m, k)).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    b = (rng.standard_normal((k, n)).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    c = np.zeros((m, n), dtype=np.float32)
+    L = G.lib()
+    for _ in range(reps):
+        rc = L.epi_gemm_bf16(a.ctypes.data, b.ctypes.data, c.ctypes.data, m, n, k, 0, 0)
+        if rc != 0:
+            raise RuntimeError(L.epi_mlp_last_error().decode())
+    print(json.dumps({"gemm": {"M": m, "N": n, "K": k, "reps": reps, "flop_per_call": 2.0 * m * n * k}}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,10 +327,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-extra", action="store_true", help="only the headline workload")
+    ap.add_argument("--surrogate-train", type=int, default=2048, help="training simulations of the surrogate leg")
+    ap.add_argument("--surrogate-eval", type=int, default=64, help="held-out points of the surrogate leg")
+    ap.add_argument("--skip-surrogate", action="store_true")
     ap.add_argument("--profile-hours", type=int, default=0,
                     help="profiling driver for rocprofv3: run this many hours as separate kernel dispatches "
                          "(no graphs, no torch), then one 64 MiB device-to-device copy as the known-size "
                          "calibration dispatch for FETCH_SIZE / WRITE_SIZE, and exit")
+    ap.add_argument("--profile-gemm", default="", help="M,N,K: profiling driver for the surrogate's MFMA GEMM")
+    ap.add_argument("--profile-gemm-reps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path; the step would shard as one "
@@ -279,6 +344,9 @@ def main():
 
     import epirust_amd as G
 
+    if args.profile_gemm:
+        profile_gemm([int(x) for x in args.profile_gemm.split(",")], args.profile_gemm_reps)
+        return
     if args.profile_hours:
         profile_driver(args, G)
         return
@@ -307,9 +375,10 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic population (deterministic Philox synthesis of the reference's Population::Auto)",
-        "config": {"workload": "EpiRust standalone hour step, %d agents, %dx%d grid, %d hours, seed %s, "
-                               "deterministic mode" % (n, g, g, run_hours, cfg.get("seed", 0)),
-                   "agents": n, "grid_size": g, "hours": run_hours,
+        "config": {"workload": "EpiRust standalone hour step, %d agents, %dx%d grid, seed %s, deterministic "
+                               "mode: %d simulated hours timed (hours 1..%d of runs of %d hours)" % (
+                                   n, g, g, cfg.get("seed", 0), hours_done, min(hours_done, run_hours), run_hours),
+                   "agents": n, "grid_size": g, "hours": run_hours, "hours_timed": hours_done,
                    "threads_per_block": args.block or tuned_block_of(cfg) or 256,
                    "kernel_ms": {"hospital_scan": scan_ms, "step": step_ms, "commit": commit_ms}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -320,17 +389,20 @@ def main():
     }
     cpu_s = 0.0 if args.no_cpu_baseline else args.cpu_seconds
     if cpu_s > 0:
-        rate, sample = oracle_rate([cfg], cpu_s)
+        # one whole simulated day per sample (hours 1..24: sleep, commute, office and evening hours)
+        rate, sample = refcpu_rate([cfg], 0, samples=3, window_hours=24)
         result["cpu_baseline"] = {"value": rate, "unit": "agent-hours/s", "cores": cpu_threads(), "kind": "port",
                                   "sample": sample}
         # the reference engine's default rayon pool is 4 threads (engine-app/src/main.rs:80-82)
-        rate4, sample4 = oracle_rate([cfg], cpu_s / 3, threads=4)
+        rate4, sample4 = refcpu_rate([cfg], 0, threads=4, samples=3, window_hours=24)
         result["cpu_baseline_4_threads"] = {"value": rate4, "unit": "agent-hours/s", "cores": 4, "kind": "port",
                                             "sample": sample4}
     if not args.skip_extra:
         result["replicates"] = bench_replicates(G, cpu_s / 3)
         result["interventions_10m"] = bench_interventions(G, cpu_s / 3)
         result["create_10m"] = bench_init(G)
+        if not args.skip_surrogate:
+            result["surrogate"] = bench_surrogate(args.surrogate_train, args.surrogate_eval, 32, 60)
     print(json.dumps(result))
 
 

@@ -155,7 +155,13 @@ struct epi_sim {
   hipEvent_t loop_event[2] = {nullptr, nullptr};  // around the hour loop of one step call
   double last_loop_ms = 0.0;                      // device time of the last step call's hours
   uint32_t last_loop_hours = 0;
+  bool loop_timed = false;  // loop_event brackets the last advance's launches
   uint32_t* pinned_active = nullptr;  // 2 x n_sims
+  SimState* pinned_state = nullptr;   // n_sims: the states a step call reads back
+  Counts7* pinned_rows = nullptr;     // n_rows: staging of the rows a step call returns
+  // The home-layout agent arrays hold the current state with the last decided sweep
+  // applied; cleared by every step call, restored by settle() before the host reads agents.
+  bool settled = true;
   uint32_t host_hour = 0;             // hours enqueued so far (every active simulation is at this hour)
   bool all_stopped = false;           // every simulation reported inactive
   std::vector<uint32_t> rows_returned;
@@ -169,6 +175,8 @@ struct epi_sim {
     for (auto& e : loop_event)
       if (e) (void)hipEventDestroy(e);
     if (pinned_active) (void)hipHostFree(pinned_active);
+    if (pinned_state) (void)hipHostFree(pinned_state);
+    if (pinned_rows) (void)hipHostFree(pinned_rows);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -212,6 +220,10 @@ int validate(const Config& c) {
   return EPI_OK;
 }
 
+// XCDs of an MI355X. The dispatcher hands a launch's workgroups to them round-robin
+// (workgroup w to XCD w % 8), in increasing order within each, and each XCD has its own L2.
+constexpr uint32_t kXcds = 8;
+
 int build_launch_tables(epi_sim* s) {
   const uint32_t B = s->shape.block;
   s->agent_blocks.clear();
@@ -220,7 +232,25 @@ int build_launch_tables(epi_sim* s) {
     const uint32_t nb = (n + B - 1) / B;
     s->desc[i].n_blocks = nb;
     s->desc[i].block_begin = static_cast<uint32_t>(s->agent_blocks.size());
-    for (uint32_t b = 0; b < nb; ++b) s->agent_blocks.push_back(BlockRef{i, b * B});
+    // Home-layout blocks in dispatch order. Housemates (ids congruent mod n_houses) sit
+    // in different blocks, one per round of ids, and read the same windows and each
+    // other's target words. Sorting the blocks by the house of their first slot makes a
+    // house's blocks neighbours; spreading the sorted list so that each XCD gets one
+    // contiguous run (table entry j runs on XCD j % 8) puts them on one XCD, close in
+    // time, sharing its L2, with that XCD's band of house rows. The order changes no
+    // result: claims are resolved by key and counts are summed.
+    const uint32_t H = std::max(s->desc[i].n_houses, 1u);
+    std::vector<uint32_t> sorted(nb);
+    for (uint32_t b = 0; b < nb; ++b) sorted[b] = b;
+    std::stable_sort(sorted.begin(), sorted.end(),
+                     [&](uint32_t a, uint32_t b) { return (a * B) % H < (b * B) % H; });
+    uint32_t start[kXcds];
+    for (uint32_t x = 0, acc = 0; x < kXcds; ++x) {
+      start[x] = acc;
+      acc += nb / kXcds + (x < nb % kXcds ? 1u : 0u);
+    }
+    for (uint32_t j = 0; j < nb; ++j)
+      s->agent_blocks.push_back(BlockRef{i, sorted[start[j % kXcds] + j / kXcds] * B});
   }
   s->shape.agent_blocks = static_cast<uint32_t>(s->agent_blocks.size());
   // Office layout: floor(B / K) whole offices of K slots per block; n_slots is a multiple
@@ -265,12 +295,17 @@ int relayout(epi_sim* s, bool to_office) {
 
 // Before the host reads agents: the home layout holds the current state (it is copied back
 // from the office layout after an hour of day 8..15) and includes the last decided sweep.
+// Step calls leave this to the first agent read after them; returns with the stream drained.
 int settle(epi_sim* s) {
-  if (s->host_hour > 0 && office_hour(s->host_hour)) {
-    const int rc = relayout(s, false);
-    if (rc != EPI_OK) return rc;
+  if (!s->settled) {
+    if (s->host_hour > 0 && office_hour(s->host_hour)) {
+      const int rc = relayout(s, false);
+      if (rc != EPI_OK) return rc;
+    }
+    HIP_TRY(launch_apply_sweep(s->v, s->shape, s->stream));
+    s->settled = true;
   }
-  HIP_TRY(launch_apply_sweep(s->v, s->shape, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
   return EPI_OK;
 }
 
@@ -288,24 +323,43 @@ int enqueue_hour(epi_sim* s, uint32_t next_hour) {
   return EPI_OK;
 }
 
+// Hours first..last of one day (24d+1 .. 24d+24). Hours of day 1..6 are one launch when
+// the handle allows the night launch and the range holds all six, and 3..6 one
+// epilogue-only launch when it allows quiet hours and the range holds all four.
+int enqueue_hours(epi_sim* s, uint32_t first, uint32_t last) {
+  for (uint32_t h = first; h <= last;) {
+    const uint32_t hd = h % 24u;
+    if (s->night && hd == 1 && last - h >= 5u) {
+      HIP_TRY(launch_night(s->v, s->shape, s->stream));
+      h += 6;
+      continue;
+    }
+    if (s->quiet_hours && hd == 3 && last - h >= 3u) {
+      HIP_TRY(launch_quiet_hours(s->v, s->shape, 4, s->stream));
+      h += 4;
+      continue;
+    }
+    const int rc = enqueue_hour(s, h);
+    if (rc != EPI_OK) return rc;
+    ++h;
+  }
+  return EPI_OK;
+}
+
+// The same hours as separate launches each (EPI_FLAG_NO_GRAPH).
+int enqueue_hours_plain(epi_sim* s, uint32_t first, uint32_t last) {
+  for (uint32_t h = first; h <= last; ++h) {
+    const int rc = enqueue_hour(s, h);
+    if (rc != EPI_OK) return rc;
+  }
+  return EPI_OK;
+}
+
 int capture_day_graph(epi_sim* s) {
   if (s->day_graph) return EPI_OK;
   hipGraph_t graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  int rc = EPI_OK;
-  for (uint32_t k = 1; k <= 24 && rc == EPI_OK; ++k) {  // hours 24d+1 .. 24d+24
-    if (s->night && k == 1) {  // hours of day 1..6: one launch
-      if (launch_night(s->v, s->shape, s->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
-      k = 6;
-      continue;
-    }
-    if (s->quiet_hours && k == 3) {  // hours of day 3..6: epilogues only, one launch
-      if (launch_quiet_hours(s->v, s->shape, 4, s->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");
-      k = 6;
-      continue;
-    }
-    rc = enqueue_hour(s, k);
-  }
+  const int rc = enqueue_hours(s, 1, 24);  // the hours of day of 24d+1 .. 24d+24
   hipError_t e = hipStreamEndCapture(s->stream, &graph);
   if (rc != EPI_OK) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -400,9 +454,15 @@ int advance_streaming(epi_sim* s, uint32_t target) {
 int advance(epi_sim* s, uint32_t target) {
   s->last_loop_ms = 0.0;
   s->last_loop_hours = 0;
+  s->loop_timed = false;
   if (s->any_empty && target > s->host_hour) return fail(EPI_ERR_PANIC, "No citizens!");
   if (s->all_stopped) return EPI_OK;
-  if (s->sink) return advance_streaming(s, target);
+  if (s->sink) {
+    const uint32_t first = s->host_hour;
+    const int rc = advance_streaming(s, target);
+    if (s->host_hour != first) s->settled = false;
+    return rc;
+  }
   const bool graph_ok = !(s->flags & EPI_FLAG_NO_GRAPH);
   const uint32_t first_hour = s->host_hour;
   HIP_TRY(hipEventRecord(s->loop_event[0], s->stream));
@@ -423,11 +483,9 @@ int advance(epi_sim* s, uint32_t target) {
       s->host_hour += 24;
     } else {
       const uint32_t end = std::min(target, (s->host_hour / 24u + 1u) * 24u);
-      while (s->host_hour < end) {
-        int rc = enqueue_hour(s, s->host_hour + 1);
-        if (rc != EPI_OK) return rc;
-        ++s->host_hour;
-      }
+      const int rc = graph_ok ? enqueue_hours(s, s->host_hour + 1, end) : enqueue_hours_plain(s, s->host_hour + 1, end);
+      if (rc != EPI_OK) return rc;
+      s->host_hour = end;
     }
     // copy the active flags of every simulation to pinned memory
     for (uint32_t i = 0; i < s->n_sims; ++i)
@@ -438,24 +496,54 @@ int advance(epi_sim* s, uint32_t target) {
     ++day_index;
   }
   HIP_TRY(hipEventRecord(s->loop_event[1], s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  float ms = 0.0f;
-  HIP_TRY(hipEventElapsedTime(&ms, s->loop_event[0], s->loop_event[1]));
-  s->last_loop_ms = ms;
+  s->loop_timed = true;
   s->last_loop_hours = s->host_hour - first_hour;
+  if (s->last_loop_hours) s->settled = false;
   return EPI_OK;
 }
 
+// After the stream has drained: device time of the last advance's hour loop.
+int record_loop_time(epi_sim* s) {
+  if (!s->loop_timed) return EPI_OK;
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, s->loop_event[0], s->loop_event[1]));
+  s->last_loop_ms = ms;
+  return EPI_OK;
+}
+
+// Rows a simulation has room for: one per hour of its loop 1..hours-1.
+uint32_t row_capacity(const epi_sim* s, uint32_t i) {
+  const uint32_t h = s->desc[i].hours;
+  return h > 1 ? h - 1 : 0u;
+}
+
+// The hours are enqueued, then the states and every simulation's rows that this call can
+// have written (at most one per hour enqueued) are copied to pinned memory behind them,
+// and the stream is drained once.
 int step_impl(epi_sim* s, uint32_t n_hours, epi_counts* rows, uint32_t row_stride, uint32_t* n_written,
               int* stopped) {
   const uint64_t want = static_cast<uint64_t>(s->host_hour) + n_hours;
   const uint32_t target = static_cast<uint32_t>(std::min<uint64_t>(want, s->max_hour));
+  const uint32_t start = s->host_hour;
   int rc = advance(s, target);
   if (rc != EPI_OK) return rc;
-  rc = settle(s);  // agent state in the home layout, with the last hour's sweep
+  const uint32_t ran = s->host_hour - start;
+  HIP_TRY(hipMemcpyAsync(s->pinned_state, s->v.state, s->n_sims * sizeof(SimState), hipMemcpyDeviceToHost,
+                         s->stream));
+  if (rows && ran) {
+    for (uint32_t i = 0; i < s->n_sims; ++i) {
+      const uint32_t from = s->rows_returned[i];
+      const uint32_t win = std::min(ran, row_capacity(s, i) - std::min(from, row_capacity(s, i)));
+      if (win == 0) continue;
+      const uint64_t at = s->desc[i].row_off + from;
+      HIP_TRY(hipMemcpyAsync(s->pinned_rows + at, s->v.rows + at, win * sizeof(Counts7), hipMemcpyDeviceToHost,
+                             s->stream));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  rc = record_loop_time(s);
   if (rc != EPI_OK) return rc;
-  rc = fetch_states(s);
-  if (rc != EPI_OK) return rc;
+  std::memcpy(s->state_h.data(), s->pinned_state, s->n_sims * sizeof(SimState));
   s->all_stopped = true;
   for (uint32_t i = 0; i < s->n_sims; ++i) s->all_stopped = s->all_stopped && !s->state_h[i].active;
   for (uint32_t i = 0; i < s->n_sims; ++i) {
@@ -463,11 +551,10 @@ int step_impl(epi_sim* s, uint32_t n_hours, epi_counts* rows, uint32_t row_strid
     if (st.error) return fail(EPI_ERR_PANIC, "device reported an error state");
     const uint32_t from = s->rows_returned[i], to = st.rows_written;
     const uint32_t k = to - from;
-    if (k > n_hours) return fail(EPI_ERR_DEVICE, "row accounting mismatch");
-    if (rows && k) {
-      HIP_TRY(hipMemcpy(rows + static_cast<size_t>(i) * row_stride, s->v.rows + s->desc[i].row_off + from,
-                        k * sizeof(Counts7), hipMemcpyDeviceToHost));
-    }
+    if (k > n_hours || k > ran) return fail(EPI_ERR_DEVICE, "row accounting mismatch");
+    if (rows && k)
+      std::memcpy(rows + static_cast<size_t>(i) * row_stride, s->pinned_rows + s->desc[i].row_off + from,
+                  k * sizeof(Counts7));
     s->rows_returned[i] = to;
     if (n_written) n_written[i] = k;
     if (stopped) stopped[i] = st.active ? 0 : 1;
@@ -738,6 +825,10 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   if (rc != EPI_OK) return rc;
   HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_active), 2 * s->n_sims * sizeof(uint32_t),
                         hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_state), s->n_sims * sizeof(SimState),
+                        hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_rows), std::max<uint64_t>(s->n_rows, 1) * sizeof(Counts7),
+                        hipHostMallocDefault));
   HIP_TRY(hipEventCreateWithFlags(&s->poll_event[0], hipEventDisableTiming));
   HIP_TRY(hipEventCreate(&s->loop_event[0]));
   HIP_TRY(hipEventCreate(&s->loop_event[1]));
@@ -861,7 +952,9 @@ int epi_stream_citizen_states(epi_sim* sim, const char* path) {
 int epi_citizen_states(epi_sim* sim, uint32_t index, epi_citizen_state* out, uint32_t cap, uint32_t* n,
                        uint32_t* hour) {
   if (!sim || index >= sim->n_sims) return fail(EPI_ERR_ARG, "bad handle or index");
-  int rc = fetch_states(sim);
+  int rc = settle(sim);
+  if (rc != EPI_OK) return rc;
+  rc = fetch_states(sim);
   if (rc != EPI_OK) return rc;
   const SimDesc& D = sim->desc[index];
   if (n) *n = D.n;
@@ -977,6 +1070,8 @@ int epi_read_agents(epi_sim* sim, uint32_t index, epi_agents_soa* dst) {
   const SimDesc& D = sim->desc[index];
   const Geometry& G = sim->geo[index];
   const uint32_t n = D.n;
+  int rc = settle(sim);
+  if (rc != EPI_OK) return rc;
   std::vector<uint32_t> pos(n), st(n), tm(n), wsa(n);
   if (n) {
     HIP_TRY(hipMemcpy(pos.data(), sim->v.pos + D.agent_off, n * 4, hipMemcpyDeviceToHost));
@@ -1141,6 +1236,7 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
     launches[k] = cnt[k];
   }
   if (hours_run) *hours_run = ran;
+  if (ran) sim->settled = false;
   rc = settle(sim);
   if (rc != EPI_OK) return rc;
   // keep row accounting consistent for later epi_step calls

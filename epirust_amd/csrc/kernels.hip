@@ -51,8 +51,16 @@ constexpr uint32_t kResidentAgents = 256u * 2048u;
 // Office-hour block claim filter: 100 cells per 10 x 10 office, at most kFilterCells words
 // of LDS (a block of up to 1024 slots covers at most 20 offices when per_office >= 52).
 constexpr uint32_t kOfficeCells = 100u, kFilterCells = 2048u, kNoCell = ~0u;
-// Target word flag (y never exceeds 32000, so bit 31 is free): the move has already won.
+// Target word flags (x and y never exceed 32000, so bits 15 and 31 are free):
+//   kTgtWon   the move has already won (office-hour block claim filter);
+//   kTgtHouse a home-layout move into a cell of the agent's own house, arbitrated in the
+//             commit among the house's agents instead of by a global claim.
+// At home-layout step hours every agent writes its target word, movers or not, so the
+// words its housemates read in the commit are this hour's.
 constexpr uint32_t kTgtWon = 1u << 31;
+constexpr uint32_t kTgtHouse = 1u << 15;
+constexpr uint32_t kTgtCell = ~(kTgtWon | kTgtHouse);
+constexpr uint32_t kTgtNone = 0u;  // a non-mover: no kTgtHouse bit, so it equals no house move
 
 // Agent id of a slot: the identity in the home layout; slot = office * per_office + k
 // holds agent k * n_offices + office in the office layout.
@@ -215,6 +223,9 @@ struct Agent {
     oy = D.office_sy + static_cast<int32_t>(orow) * 10;
   }
   __device__ __forceinline__ Rect home() const { return Rect{hx, hy, hx + 1, hy + 1}; }
+  __device__ __forceinline__ bool in_house(int32_t cx, int32_t cy) const {
+    return cx >= hx && cx <= hx + 1 && cy >= hy && cy <= hy + 1;
+  }
   __device__ __forceinline__ Rect work() const {  // non-workers' work location is home
     return ws() == WS_NA ? home() : Rect{ox, oy, ox + 9, oy + 9};
   }
@@ -333,7 +344,7 @@ struct Agent {
     const int k = __builtin_ctz(cand);
     tx = lx + moore_dx(k);
     ty = ly + moore_dy(k);
-    if (precheck) seen_claim = claims[cell_index(D, tx, ty)];  // overlaps the disease update
+    if (precheck && !in_house(tx, ty)) seen_claim = claims[cell_index(D, tx, ty)];  // overlaps the disease update
   }
 
   // default_disease_handler.rs:64-86 (OR over fixed per-neighbour draws)
@@ -711,7 +722,7 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   bool moves = false;
   uint32_t key = 0, w_out = 0, w_in = 0, tgt_word = 0, office_cell = kNoCell;
   bool whole_office = false;  // the claimed cell's office has all its slots in this block
-  bool first_of_house = false;  // the claim is of the agent's own house by its first agent
+  bool house_move = false;    // a home-layout move into the agent's own house
   const uint64_t gi = D.agent_off + slot;
   uint64_t ci = 0;
   // Office hours: the offices this block's slots belong to (j_lo .. j_lo + n_off - 1), each
@@ -742,11 +753,10 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     const uint64_t gid = D.agent_off + local;  // tm and wsa stay in the home layout (by id)
     a.tm_arr = v.tm + gid;
     a.claims = v.claim + D.cell_off;
-    // At office hours the block claim filter below resolves the contended cells instead.
-    // At the other hours a local move stays in the agent's 2x2 house, whose other agents
-    // (ids congruent mod n_houses) are its only rivals: the first of a house (id below
-    // n_houses) has no lower-id rival to find.
-    a.precheck = slot >= kResidentAgents && !v.office && local >= D.n_houses;
+    // At office hours the block claim filter below resolves the contended cells instead,
+    // and at the other hours moves into the agent's own house need no claim (below): the
+    // pre-check serves the remaining local moves (transport area, hospital).
+    a.precheck = slot >= kResidentAgents && !v.office;
     const uint32_t w0 = w_stored;
     const bool staff = a.ws() == WS_STAFF;
     a.wsa = staff ? v.wsa[gid] : 0u;
@@ -793,12 +803,14 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
                     static_cast<uint32_t>(a.tx - a.ox);
       whole_office = j * D.per_office >= first && (j + 1u) * D.per_office <= first + span;
     }
-    // Home-layout hours: a house's cells are claimed only by its own agents (ids congruent
-    // mod n_houses), and the first of them (id below n_houses) has the lowest id, so its
-    // claim of a cell of its own house wins. It still claims, so that its housemates lose,
-    // but flags its target word so that the commit need not read the claim back.
-    first_of_house = !v.office && local < D.n_houses && a.tx >= a.hx && a.tx <= a.hx + 1 && a.ty >= a.hy &&
-                     a.ty <= a.hy + 1;
+    // Home-layout hours (16..23, 0..7): only a house's own agents (ids congruent mod
+    // n_houses) choose cells of that house. Local moves stay in the current area, which is
+    // home for everyone but public-transport users (transport area) and hospital staff;
+    // the teleports into a house (going home at 16, recovered going home at 23, the
+    // hospital fallback at 0) are its own agents'; no one has the housing area then. So
+    // the lowest-id claimant of such a cell is found among the agent's lower housemates,
+    // whose target words the commit reads: no global claim is made.
+    house_move = !v.office && moves && a.in_house(a.tx, a.ty);
     if (tm_dirty) v.tm[gid] = a.tm;
     if (staff && a.wsa != wsa0) v.wsa[gid] = a.wsa;
   }
@@ -821,14 +833,11 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   if (valid) {
     if (moves) {
       w_out |= F_MOVE;
-      if (whole_office) {
-        tgt_word |= kTgtWon;
-      } else {
-        atomicMin(&v.claim[ci], key);
-        if (first_of_house) tgt_word |= kTgtWon;
-      }
-      v.tgt[gi] = tgt_word;
+      if (whole_office) tgt_word |= kTgtWon;
+      else if (house_move) tgt_word |= kTgtHouse;
+      else atomicMin(&v.claim[ci], key);
     }
+    if (moves || !v.office) v.tgt[gi] = moves ? tgt_word : kTgtNone;
     if (w_out != w_in) v.st[gi] = w_out;
   }
 
@@ -956,22 +965,37 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
   const uint32_t local = agent_of_slot(v, D, slot);  // the agent id
   if (slot < D.n_slots && local < D.n && (!v.office || threadIdx.x < D.office_span)) {
     const uint64_t gi = D.agent_off + slot;
-    // st, pos and tgt are loaded together so a mover pays two dependent round trips
-    // (these three, then its claim), not four.
+    // st, pos and tgt are loaded together, and at home-layout hours with them the target
+    // words of the agent's lower housemates (ids local - k * n_houses, k = 1..3: a house
+    // holds at most 4 agents, grid.rs:133-142), consecutive for consecutive ids. A house
+    // move then pays one round trip; a claimed move two (these words, then its claim).
     uint32_t w = v.st[gi];
     uint32_t p = v.pos[gi];
     uint32_t t = v.tgt[gi];
+    uint32_t m1 = kTgtNone, m2 = kTgtNone, m3 = kTgtNone;
+    if (!v.office) {  // uniform
+      const uint32_t lower = fast_div(local, D.fd_houses);  // lower housemates
+      const uint32_t H = D.fd_houses.d;
+      if (lower >= 1) m1 = v.tgt[gi - H];
+      if (lower >= 2) m2 = v.tgt[gi - 2ull * H];
+      if (lower >= 3) m3 = v.tgt[gi - 3ull * H];
+      loaded_together(m1, m2, m3);
+    }
     loaded_together(w, p, t);
-    const bool won = (t & kTgtWon) != 0u;  // decided by the office-hour claim filter
     if (w & (F_MOVE | F_DIRTY)) {
       uint8_t* occ = v.occ + D.cell_off;
       const uint64_t old_ci = cell_index(D, static_cast<int32_t>(p & 0xFFFFu), static_cast<int32_t>(p >> 16));
       const uint8_t byte = grid_byte(D, w);
       bool moved = false;
       if (w & F_MOVE) {
-        t &= ~kTgtWon;
+        // won: decided by the office-hour claim filter; a house move wins unless a lower
+        // housemate chose the same cell (the same word, kTgtHouse included); any other
+        // move reads back its claim
+        const bool house = (t & kTgtHouse) != 0u;
+        const bool won = house ? (m1 != t && m2 != t && m3 != t) : (t & kTgtWon) != 0u;
+        t &= kTgtCell;
         const uint64_t ci = cell_index(D, static_cast<int32_t>(t & 0xFFFFu), static_cast<int32_t>(t >> 16));
-        if (won || v.claim[D.cell_off + ci] == claim_key(h, local)) {
+        if (won || (!house && v.claim[D.cell_off + ci] == claim_key(h, local))) {
           occ[old_ci] = 0;
           occ[ci] = byte;
           v.pos[gi] = t;

@@ -2,13 +2,15 @@
 //
 // GEMM: C[M x N] = A[M x K] * B[K x N] with arbitrary element strides for A and B (so
 // one kernel serves X*W, X^T*dZ and dZ*W^T), a 64x64 block tile, BK = 32, four 64-lane
-// waves each owning a 32x32 sub-tile computed with v_mfma_f32_32x32x8_bf16 (A and B
-// staged through LDS), and a fused epilogue: + bias[n], optional GELU (tanh form),
-// optional FP32 and/or bf16 stores, optional store of the pre-activation.
+// waves each owning a 32x32 sub-tile computed with v_mfma_f32_32x32x16_bf16 (gfx950; A
+// and B staged through LDS as 16-byte vectors, two buffers, the next tile prefetched into
+// registers during the current one's MFMAs), and a fused epilogue: + bias[n], optional
+// GELU (tanh form), optional FP32 and/or bf16 stores, optional store of the
+// pre-activation.
 //
-// v_mfma_f32_32x32x8_bf16 lane layout (64 lanes, wave64):
-//   A (32x8): lane l holds row l%32, k = 4*(l/32) .. +3 (four bf16 in 64 bits)
-//   B (8x32): lane l holds col l%32, k = 4*(l/32) .. +3
+// v_mfma_f32_32x32x16_bf16 lane layout (64 lanes, wave64):
+//   A (32x16): lane l holds row l%32, k = 8*(l/32) .. +7 (eight bf16 in 128 bits)
+//   B (16x32): lane l holds col l%32, k = 8*(l/32) .. +7
 //   D (32x32, 16 floats / lane): lane l, element r -> row 8*(r/4) + 4*(l/32) + r%4, col l%32
 #include <hip/hip_runtime.h>
 
@@ -20,10 +22,58 @@
 namespace epi {
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBM = 64, kBN = 64, kBK = 32;
+constexpr int kLdsK = kBK + 8;  // LDS row: 32 k of one m (or n) plus 8 of padding, 80 bytes
+
+// How one 64 x 32 operand tile (rows r = m or n, columns k) is read from global memory.
+enum : int { TILE_K_CONTIG = 0, TILE_R_CONTIG = 1, TILE_ELEMENT = 2 };
+
+// Element (r, k) of an operand is X[r * ld_r + k * ld_k]. A whole tile inside the matrix
+// whose contiguous dimension is 16-byte aligned is read as 16-byte vectors (8 bf16 per
+// thread, 256 threads): along k (ld_k == 1) or along r (ld_r == 1). Otherwise, at ragged
+// edges or odd strides, element by element with bounds checks.
+__device__ __forceinline__ int tile_mode(const uint16_t* X, int64_t ld_r, int64_t ld_k, int r0, int k0, int R,
+                                         int K) {
+  const bool full = r0 + 64 <= R && k0 + kBK <= K;
+  const bool aligned = (reinterpret_cast<uintptr_t>(X) & 15u) == 0;
+  if (full && aligned && ld_k == 1 && (ld_r & 7) == 0) return TILE_K_CONTIG;
+  if (full && aligned && ld_r == 1 && (ld_k & 7) == 0) return TILE_R_CONTIG;
+  return TILE_ELEMENT;
+}
+
+__device__ __forceinline__ u16x8 load_tile(const uint16_t* X, int64_t ld_r, int64_t ld_k, int r0, int k0, int R,
+                                           int K, int mode, int tid) {
+  u16x8 v;
+  if (mode == TILE_K_CONTIG) {  // thread: row tid / 4, k (tid % 4) * 8 .. +7
+    v = *reinterpret_cast<const u16x8*>(X + static_cast<int64_t>(r0 + (tid >> 2)) * ld_r + k0 + (tid & 3) * 8);
+  } else if (mode == TILE_R_CONTIG) {  // thread: k tid / 8, rows (tid % 8) * 8 .. +7
+    v = *reinterpret_cast<const u16x8*>(X + static_cast<int64_t>(k0 + (tid >> 3)) * ld_k + r0 + (tid & 7) * 8);
+  } else {  // the TILE_K_CONTIG assignment, element by element
+    const int r = r0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = kb + q;
+      v[q] = (r < R && k < K) ? X[static_cast<int64_t>(r) * ld_r + static_cast<int64_t>(k) * ld_k] : 0;
+    }
+  }
+  return v;
+}
+
+// LDS tile s[r][k] (k contiguous): one 16-byte store, or eight 2-byte stores for a tile
+// read along r.
+__device__ __forceinline__ void store_tile(uint16_t (*s)[kLdsK], const u16x8& v, int mode, int tid) {
+  if (mode == TILE_R_CONTIG) {
+    const int k = tid >> 3, rb = (tid & 7) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[rb + q][k] = v[q];
+  } else {
+    *reinterpret_cast<u16x8*>(&s[tid >> 2][(tid & 3) * 8]) = v;
+  }
+}
 
 // fp32 -> bf16, round to nearest even (NaN kept quiet).
 __device__ __forceinline__ uint16_t to_bf16(float f) {
@@ -47,40 +97,45 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * du;
 }
 
+// C tile 64 x 64 per block of four wave64s (2 x 2), each a 32 x 32 sub-tile accumulated
+// with v_mfma_f32_32x32x16_bf16 (gfx950): per 32-wide k step, two MFMAs whose A operand is
+// row (lane % 32) with k 8 * (lane / 32) .. +7, read from LDS as one 16-byte vector, and B
+// likewise by column. The next k tile's global loads are issued before the current one's
+// MFMAs (register prefetch), into the other of two LDS buffers, so each k step has one
+// barrier.
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-  __shared__ uint16_t sA[kBK][kBM + 4];  // k-major: sA[k][m]
-  __shared__ uint16_t sB[kBK][kBN + 4];  // sB[k][n]
+  __shared__ __align__(16) uint16_t sA[2][kBM][kLdsK];
+  __shared__ __align__(16) uint16_t sB[2][kBN][kLdsK];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   f32x16 acc = {0};
 
+  int ma = tile_mode(g.A, g.lda_m, g.lda_k, m0, 0, g.M, g.K);
+  int mb = tile_mode(g.B, g.ldb_n, g.ldb_k, n0, 0, g.N, g.K);
+  u16x8 va = load_tile(g.A, g.lda_m, g.lda_k, m0, 0, g.M, g.K, ma, tid);
+  u16x8 vb = load_tile(g.B, g.ldb_n, g.ldb_k, n0, 0, g.N, g.K, mb, tid);
+  int buf = 0;
   for (int k0 = 0; k0 < g.K; k0 += kBK) {
-    // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int e = tid + r * 256;
-      const int am = e % kBM, ak = e / kBM;  // consecutive threads -> consecutive m
-      const int gm = m0 + am, gk = k0 + ak;
-      sA[ak][am] = (gm < g.M && gk < g.K) ? g.A[static_cast<int64_t>(gm) * g.lda_m + static_cast<int64_t>(gk) * g.lda_k] : 0;
-      const int bn = e % kBN, bk = e / kBN;
-      const int gn = n0 + bn, gk2 = k0 + bk;
-      sB[bk][bn] = (gn < g.N && gk2 < g.K) ? g.B[static_cast<int64_t>(gk2) * g.ldb_k + static_cast<int64_t>(gn) * g.ldb_n] : 0;
-    }
+    store_tile(sA[buf], va, ma, tid);
+    store_tile(sB[buf], vb, mb, tid);
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < kBK; kk += 8) {
-      const int kb = kk + 4 * (lane >> 5);
-      bf16x4 a, b;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        a[q] = static_cast<short>(sA[kb + q][wm + (lane & 31)]);
-        b[q] = static_cast<short>(sB[kb + q][wn + (lane & 31)]);
-      }
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    const int k1 = k0 + kBK;
+    if (k1 < g.K) {  // uniform: prefetch the next tile while this one is multiplied
+      ma = tile_mode(g.A, g.lda_m, g.lda_k, m0, k1, g.M, g.K);
+      mb = tile_mode(g.B, g.ldb_n, g.ldb_k, n0, k1, g.N, g.K);
+      va = load_tile(g.A, g.lda_m, g.lda_k, m0, k1, g.M, g.K, ma, tid);
+      vb = load_tile(g.B, g.ldb_n, g.ldb_k, n0, k1, g.N, g.K, mb, tid);
     }
-    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 16) {
+      const int kq = kk + 8 * (lane >> 5);
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[buf][wm + (lane & 31)][kq]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[buf][wn + (lane & 31)][kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    buf ^= 1;
   }
   // epilogue
   const int col = n0 + wn + (lane & 31);

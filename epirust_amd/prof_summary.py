@@ -2,6 +2,8 @@
 
     python -m epirust_amd.prof_summary trace <kernel_stats.csv> [<kernel_trace.csv> --hours H] [--command "..."]
     python -m epirust_amd.prof_summary traffic <fetch.csv> <write.csv> --agents N --grid G --hours H [--command "..."]
+    python -m epirust_amd.prof_summary counters <counter_collection.csv>... [--command "..."]
+    python -m epirust_amd.prof_summary gemm <kernel_stats.csv> --shape M,N,K [--command "..."]
 
 `trace` turns a `--kernel-trace --stats` kernel_stats.csv into a markdown table (calls,
 average / min / max duration per kernel) and, given the kernel_trace.csv, a table of the
@@ -141,9 +143,66 @@ def traffic(fetch_csv, write_csv, agents, grid, hours, command):
                                      "hours": hours, "fetch_scale": f_scale, "write_scale": w_scale}
 
 
+def counters(paths, command, clock_ghz=2.4):
+    """Per-kernel means per dispatch of the counters in one or more counter_collection.csv
+    files (each a separate --pmc pass), with derived fractions where their inputs are
+    present: VALU busy = SQ_ACTIVE_INST_VALU x 4 cycles over SQ_BUSY_CYCLES x the SIMDs
+    per SQ; waves waiting = SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES; L2 hit rate =
+    TCC_HIT / (TCC_HIT + TCC_MISS); atomics past the L2 = TCC_EA0_ATOMIC / TCC_ATOMIC."""
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    n = collections.defaultdict(set)
+    for path in paths:
+        for r in csv.DictReader(open(path)):
+            k = short(r["Kernel_Name"])
+            if k not in HOUR_KERNELS:
+                continue
+            per[k][r["Counter_Name"]] += float(r["Counter_Value"])
+            n[(k, r["Counter_Name"])].add(int(r["Dispatch_Id"]))
+    names = sorted({c for d in per.values() for c in d})
+    out = ["# Hardware counters per dispatch", "", "Commands:", ""] + ["    " + c for c in command.split(" && ")] + [
+        "", "| kernel | " + " | ".join(names) + " | derived |", "|---" * (len(names) + 2) + "|"]
+    for k in HOUR_KERNELS:
+        if k not in per:
+            continue
+        mean = {c: per[k][c] / max(len(n[(k, c)]), 1) for c in per[k]}
+        der = []
+        if "SQ_WAVES" in mean and "SQ_INSTS_VALU" in mean:
+            der.append("%.0f VALU and %.0f SALU instructions per wave" % (
+                mean["SQ_INSTS_VALU"] / mean["SQ_WAVES"], mean.get("SQ_INSTS_SALU", 0) / mean["SQ_WAVES"]))
+        if "SQ_WAIT_INST_ANY" in mean and "SQ_WAVE_CYCLES" in mean:
+            der.append("waves waiting %.0f%% of their cycles" % (100 * mean["SQ_WAIT_INST_ANY"] / mean["SQ_WAVE_CYCLES"]))
+        if "SQ_ACTIVE_INST_VALU" in mean and "SQ_BUSY_CYCLES" in mean:
+            # SQ_BUSY_CYCLES sums the busy cycles of the 32 shader engines (32 SIMDs each; it
+            # reads ~32 x the kernel's duration in cycles); a wave64 VALU instruction
+            # occupies its SIMD 4 cycles
+            der.append("VALU busy %.0f%% of SIMD cycles" % (
+                100 * 4 * mean["SQ_ACTIVE_INST_VALU"] / (mean["SQ_BUSY_CYCLES"] * 32)))
+        if "TCC_HIT_sum" in mean and "TCC_MISS_sum" in mean:
+            der.append("L2 hit %.0f%%" % (100 * mean["TCC_HIT_sum"] / (mean["TCC_HIT_sum"] + mean["TCC_MISS_sum"])))
+        if mean.get("TCC_ATOMIC_sum"):
+            der.append("atomics past the L2 %.0f%%" % (100 * mean.get("TCC_EA0_ATOMIC_sum", 0) / mean["TCC_ATOMIC_sum"]))
+        out.append("| %s | " % k + " | ".join("%.4g" % mean.get(c, 0) for c in names) + " | " + "; ".join(der) + " |")
+    return "\n".join(out) + "\n"
+
+
+def gemm(stats_csv, m, n, k, command, peak_tflops=2500.0):
+    """k_gemm's mean duration from a kernel_stats.csv, as TFLOP/s and a fraction of the
+    bf16 dense MFMA peak."""
+    for r in csv.DictReader(open(stats_csv)):
+        if short(r["Name"]) == "k_gemm":
+            us = float(r["AverageNs"]) / 1e3
+            tf = 2.0 * m * n * k / (us * 1e-6) / 1e12
+            return ("# k_gemm (bf16 MFMA, fp32 accumulation)\n\nCommand: `%s`\n\n"
+                    "| M x N x K | calls | mean µs | min µs | TFLOP/s | fraction of %.0f TFLOP/s |\n|---|---|---|---|---|---|\n"
+                    "| %d x %d x %d | %s | %.2f | %.2f | %.1f | %.4f |\n" % (
+                        command, peak_tflops, m, n, k, r["Calls"], us, float(r["MinNs"]) / 1e3, tf, tf / peak_tflops))
+    raise SystemExit("no k_gemm in %s" % stats_csv)
+
+
 def main(argv):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "traffic"])
+    ap.add_argument("mode", choices=["trace", "traffic", "counters", "gemm"])
+    ap.add_argument("--shape", default="4096,512,512", help="gemm: M,N,K")
     ap.add_argument("paths", nargs="+")
     ap.add_argument("--command", default="")
     ap.add_argument("--agents", type=int, default=1000000)
@@ -154,6 +213,11 @@ def main(argv):
     a = ap.parse_args(argv)
     if a.mode == "trace":
         text = trace(a.paths[0], a.command, a.paths[1] if len(a.paths) > 1 else None, a.hours)
+    elif a.mode == "counters":
+        text = counters(a.paths, a.command)
+    elif a.mode == "gemm":
+        m, n, k = (int(x) for x in a.shape.split(","))
+        text = gemm(a.paths[0], m, n, k, a.command)
     else:
         text, js = traffic(a.paths[0], a.paths[1], a.agents, a.grid, a.hours, a.command)
         if a.json:

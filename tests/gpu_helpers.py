@@ -13,7 +13,17 @@ def say(msg):
     sys.stdout.flush()
 
 
+def assert_one_agent_per_cell(agents, where=""):
+    """No two agents share a cell (the occupancy map holds one citizen per key,
+    allocation_map.rs:44-48): the invariant the housemate arbitration and the block claim
+    filter must keep."""
+    key = np.asarray(agents["x"]).astype(np.int64) * 65536 + np.asarray(agents["y"])
+    if len(np.unique(key)) != len(key):
+        raise AssertionError("%s: %d agents share a cell" % (where, len(key) - len(np.unique(key))))
+
+
 def assert_agents_equal(gpu, cpu, where=""):
+    assert_one_agent_per_cell(gpu, where)
     for f in FIELDS:
         a, b = np.asarray(gpu[f]), np.asarray(cpu[f])
         if a.shape != b.shape or not np.array_equal(a, b):

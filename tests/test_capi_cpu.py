@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "epirust_gpu.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(epi_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(epi_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_header_declares_the_boundary():
@@ -24,6 +24,12 @@ def test_header_declares_the_boundary():
     for n in ("epi_create", "epi_step", "epi_run", "epi_read_agents", "epi_batch_create", "epi_last_error",
               "epi_destroy"):
         assert n in names
+
+
+def test_rust_binding_declares_every_entry_point():
+    """epirust_amd/rust/epirust-hip-sys/src/lib.rs mirrors the header one for one."""
+    text = open(os.path.join(ROOT, "epirust_amd", "rust", "epirust-hip-sys", "src", "lib.rs")).read()
+    assert sorted(set(re.findall(r"pub fn (epi_[a-z_0-9]+)\(", text))) == declared_functions()
 
 
 def test_library_exports_every_declared_symbol():

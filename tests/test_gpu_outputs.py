@@ -1,7 +1,9 @@
 """The drop-in output files, byte for byte.
 
 What the reference writes for a standalone run (engine-app -c CFG -o DIR):
-  * DIR/output/ is created when absent, errors ignored (engine/src/utils/util.rs:28-35);
+  * DIR/output/ is created when absent, errors ignored (engine/src/utils/util.rs:28-35):
+    fs::create_dir, so DIR itself must exist, or the CSV write fails with "Failed to write
+    to file" (csv_service.rs:66);
   * the file stem is simulation_<engine id>_<UTC [year]-[month]-[day]T[hour]:[minute]:[second]>
     (util.rs:36-43), engine id "0" in standalone mode (engine_app.rs:30);
   * the counts CSV is <stem>.csv (epidemiology_simulation.rs:144-147), written by the csv
@@ -30,7 +32,7 @@ import pytest
 import epirust_amd as G
 from oracle import epicpu as O
 
-from gpu_helpers import interventions_config, say
+from tests.gpu_helpers import interventions_config, say
 
 pytestmark = pytest.mark.gpu
 
@@ -80,6 +82,7 @@ def test_epi_run_files_match_oracle_bytes(cfg1, tmp_path, monkeypatch):
     monkeypatch.chdir(tmp_path)
     monkeypatch.delenv("EPI_OUTPUT_DIR", raising=False)
     say("[outputs] epi_run, %d agents, %d hours" % (cfg["population"]["Auto"]["number_of_agents"], cfg["hours"]))
+    (tmp_path / "run").mkdir()
     sim = G.Simulation(cfg)
     path = sim.run(str(tmp_path / "run"))
     sim.close()
@@ -104,6 +107,7 @@ def test_epi_run_without_interventions_writes_empty_report(cfg1, tmp_path, monke
     assert ev == [] and json_text == "[]"
     monkeypatch.chdir(tmp_path)
     monkeypatch.delenv("EPI_OUTPUT_DIR", raising=False)
+    (tmp_path / "out").mkdir()
     path = G.EngineApp.start_standalone(cfg, "out")
     d, stem = split_outputs(path)
     assert d == "./out/output"  # EPI_OUTPUT_DIR defaults to "."
@@ -120,7 +124,7 @@ def test_output_dir_env_prefixes_relative_paths(cfg1, tmp_path, monkeypatch):
     work = tmp_path / "cwd"
     prefix = tmp_path / "prefix"
     (prefix / "rel" / "output").mkdir(parents=True)
-    work.mkdir()
+    (work / "rel").mkdir(parents=True)
     monkeypatch.chdir(work)
     monkeypatch.setenv("EPI_OUTPUT_DIR", str(prefix))
     sim = G.Simulation(cfg)
@@ -140,6 +144,7 @@ def test_absolute_output_dir_ignores_env(cfg1, tmp_path, monkeypatch):
     csv_text, _, _, _ = oracle_outputs(cfg, cfg["hours"])
     monkeypatch.chdir(tmp_path)
     monkeypatch.setenv("EPI_OUTPUT_DIR", str(tmp_path / "elsewhere"))
+    (tmp_path / "abs").mkdir()
     sim = G.Simulation(cfg)
     path = sim.run(str(tmp_path / "abs"))
     sim.close()
@@ -160,6 +165,7 @@ def test_cli_executable_end_to_end(cfg1, tmp_path):
     env = dict(os.environ)
     env.pop("EPI_OUTPUT_DIR", None)
     say("[outputs] epirust-gpu executable")
+    (tmp_path / "cli").mkdir()
     p = subprocess.run([G.EXE_PATH, "-c", str(cfg_path), "-m", "standalone", "-t", "4", "-o", str(tmp_path / "cli")],
                        cwd=str(tmp_path), env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
                        timeout=120)
@@ -186,8 +192,25 @@ def test_cli_entry_point_matches_executable(cfg1, tmp_path, monkeypatch):
     cfg_path.write_text(json.dumps(cfg))
     monkeypatch.chdir(tmp_path)
     monkeypatch.delenv("EPI_OUTPUT_DIR", raising=False)
+    (tmp_path / "abi").mkdir()
     assert G.cli(["--config", str(cfg_path), "--output-dir=" + str(tmp_path / "abi")]) == 0
     out_dir = tmp_path / "abi" / "output"
     csvs = [f for f in os.listdir(str(out_dir)) if f.endswith(".csv")]
     assert len(csvs) == 1
     assert (out_dir / csvs[0]).read_bytes() == csv_text.encode()
+
+
+def test_missing_output_dir_fails_like_the_reference(cfg1, tmp_path, monkeypatch):
+    """create_out_dir_if_not_present makes only the last component (fs::create_dir, its error
+    ignored), so with DIR missing the CSV write fails: the reference panics with "Failed to
+    write to file" (csv_service.rs:66); the library returns EPI_ERR_IO with that text."""
+    cfg = run_config(cfg1)
+    cfg["hours"] = 50
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("EPI_OUTPUT_DIR", raising=False)
+    sim = G.Simulation(cfg)
+    with pytest.raises(G.EpiError) as ei:
+        sim.run(str(tmp_path / "missing" / "dir"))
+    sim.close()
+    assert ei.value.code == 5 and ei.value.message.startswith("Failed to write to file: ")
+    assert not (tmp_path / "missing").exists()

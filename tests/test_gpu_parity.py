@@ -9,7 +9,7 @@ import pytest
 
 import epirust_amd as G
 from oracle import epicpu as O
-from tests.gpu_helpers import assert_agents_equal, interventions_config, say
+from tests.gpu_helpers import assert_agents_equal, assert_one_agent_per_cell, interventions_config, say
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -224,10 +224,49 @@ def test_cfg3_10m_first_day_bit_exact():
     run_lockstep(cfg, 30, "cfg3", every=15)
 
 
+def test_cfg3_full_run_matches_oracle_fixture():
+    """The whole 4,799-hour run at 10,000,000 agents with lockdown, hospital growth and 170
+    daily vaccinations: every row and every intervention event equal the oracle's, and the
+    digest of all agents' state equals the oracle's after the hour that follows each
+    lockdown / unlock / hospital-growth event, after hours 504, 505 and 721, and after the
+    last hour (tests/golden/make_golden.py, cfg3)."""
+    path = os.path.join(HERE, "golden", "cfg3_10m_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated")
+    gold = np.load(path)
+    from tests.golden.make_golden import state_digest
+    cfg = json.load(open(os.path.join(HERE, "golden", "configs", "cfg3_10m_interventions.json")))
+    say("[parity] cfg3 full run vs fixture, digests at %s" % gold["digest_hours"].tolist())
+    gpu = G.Simulation(cfg)
+    rows, h = [], 0
+    for target, digest in zip(gold["digest_hours"].tolist(), gold["digests"].tolist()):
+        r, stopped = gpu.step(target - h)
+        rows.append(r)
+        h += len(r)
+        assert h == target, (h, target)
+        a = gpu.agents()
+        assert_one_agent_per_cell(a, "cfg3 hour %d" % h)
+        assert state_digest(a) == digest, "cfg3 state digest after hour %d" % h
+        say("[parity] cfg3 digest after hour %d ok" % h)
+    rows = np.concatenate(rows)
+    assert np.array_equal(rows, gold["rows"])
+    ev = np.array(gpu.events(), dtype=np.uint32).reshape(-1, 2)
+    assert np.array_equal(ev, gold["events"])
+    kinds = ev[:, 1].tolist()
+    assert kinds.count(0) == 170 and kinds.count(1) >= 1 and kinds.count(3) == 1
+    gpu.close()
+
+
 def test_cfg4_batched_replicates_match_their_own_runs():
     """The 64-replicate batch in one launch: 8 of its replicates compared with their own
-    oracle runs (rows and agents) after 72 hours; all 64 checked for count invariants
-    over the full 2,399 hours."""
+    oracle runs (rows and agents) after 72 hours, then every row of all 64 replicates over
+    the whole 2,399 hours and each one's final state digest against its own oracle run
+    (tests/golden/make_golden.py, cfg4)."""
+    path = os.path.join(HERE, "golden", "cfg4_replicates_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("fixture not generated")
+    gold = np.load(path)
+    from tests.golden.make_golden import state_digest
     cfgs = json.load(open(os.path.join(HERE, "golden", "configs", "cfg4_replicates.json")))
     say("[parity] cfg4 batch of %d, first 72 hours" % len(cfgs))
     batch = G.Batch(cfgs)
@@ -238,16 +277,16 @@ def test_cfg4_batched_replicates_match_their_own_runs():
         assert np.array_equal(rows[i], c_rows), "replicate %d rows" % i
         assert_agents_equal(batch.agents(i), cpu.agents(), "replicate %d" % i)
         cpu.close()
-    say("[parity] cfg4 full run invariants")
+    say("[parity] cfg4 full run vs fixture")
     rest, stopped = batch.step(2399 - 72)
     for i, c in enumerate(cfgs):
         r = np.concatenate([rows[i], rest[i]]) if len(rest[i]) else rows[i]
-        n = c["population"]["Auto"]["number_of_agents"]
-        assert (r[:, 1:].sum(axis=1) == n).all()
-        assert (np.diff(r[:, 1].astype(np.int64)) <= 0).all()          # S never grows
-        assert (np.diff(r[:, 5].astype(np.int64) + r[:, 6]) >= 0).all()  # R + D never shrink
-        assert (np.diff(r[:, 0].astype(np.int64)) == 1).all() and r[0, 0] == 1
+        k = int(gold["n_rows"][i])
+        assert len(r) == k and np.array_equal(r, gold["rows"][i, :k]), "replicate %d rows" % i
         assert stopped[i]
+        a = batch.agents(i)
+        assert_one_agent_per_cell(a, "replicate %d" % i)
+        assert state_digest(a) == str(gold["digests"][i]), "replicate %d final state" % i
 
 
 def test_csv_population_bit_exact(cfg1, tmp_path):
@@ -284,3 +323,34 @@ def test_sweeps_at_sleep_hours_and_at_the_last_hour(cfg1):
     cpu.close()
     h = run_lockstep(cfg, 80, "sweeps hourly", every=1)
     assert h == 80
+
+
+@pytest.mark.parametrize("chunk", [20, 7])
+def test_partial_days_match_day_graphs_and_plain_launches(cfg1, chunk):
+    """Step calls that end inside a day enqueue their hours directly, with hours of day 1..6
+    (or 3..6) as one launch when the call holds all of them; EPI_FLAG_NO_GRAPH launches
+    every hour on its own. Both equal the oracle, with a lockdown decided at an hour of
+    day 0 and vaccinations at hours of day 0 and 6 (so the night launch is allowed)."""
+    cfg = interventions_config(cfg1)
+    cfg["hours"] = 24 * 14
+    cfg["interventions"] = [{"Lockdown": {"at_number_of_infections": 30, "essential_workers_population": 0.2}},
+                            {"Vaccinate": {"at_hour": 96, "percent": 0.3}},
+                            {"Vaccinate": {"at_hour": 150, "percent": 0.2}}]
+    say("[parity] partial days, %d-hour calls" % chunk)
+    for flags in (G.EPI_FLAG_DETERMINISTIC, G.EPI_FLAG_DETERMINISTIC | G.EPI_FLAG_NO_GRAPH):
+        gpu = G.Simulation(cfg, flags=flags)
+        cpu = O.OracleSim(cfg)
+        h = 0
+        while h < cfg["hours"] - 1:
+            g_rows, g_stop = gpu.step(chunk)
+            c_rows, c_stop = cpu.step(chunk)
+            assert np.array_equal(g_rows, c_rows), "flags %d: rows after hour %d" % (flags, h)
+            assert g_stop == c_stop
+            h += len(c_rows)
+            if h % (3 * chunk) == 0 or c_stop:
+                assert_agents_equal(gpu.agents(), cpu.agents(), "flags %d hour %d" % (flags, h))
+            if c_stop:
+                break
+        assert gpu.events() == cpu.events()
+        gpu.close()
+        cpu.close()
