@@ -86,6 +86,8 @@ SYMBOLS = {
     "epi_citizen_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
                                           _P(ctypes.c_uint32), _P(ctypes.c_uint32)]),
     "epi_stream_citizen_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "epi_citizen_states_text_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                                       ctypes.c_void_p]),
     "epi_last_error": (ctypes.c_char_p, []),
     "epi_destroy": (None, [ctypes.c_void_p]),
     "epi_cli_main": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_char_p)]),

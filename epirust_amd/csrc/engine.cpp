@@ -581,6 +581,10 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   }
   int rc = check_device();
   if (rc != EPI_OK) return rc;
+  {  // set-up of the no-allocation entry point: this device's copy of its scratch array
+    unsigned long long* scratch = nullptr;
+    HIP_TRY(noalloc_scratch(&scratch));
+  }
 
   // geometry and the reference's panic checks on the host; the agents themselves are
   // synthesized on the device unless EPI_FLAG_HOST_INIT asks for the host restatement
@@ -947,6 +951,35 @@ int epi_stream_citizen_states(epi_sim* sim, const char* path) {
   int rc = close_sink(sim);
   if (rc != EPI_OK || !path) return rc;
   return open_sink(sim, path);
+}
+
+int epi_citizen_states_text_noalloc(epi_sim* sim, char* d_text, uint64_t cap, uint64_t* d_len) {
+  if (!sim || !d_text || !d_len) return fail(EPI_ERR_ARG, "sim, d_text or d_len is null");
+  if (sim->n_sims != 1) return fail(EPI_ERR_ARG, "the citizen-state text needs a single-simulation handle");
+  const SimDesc& D = sim->desc[0];
+  if (D.n > kNoAllocMaxAgents)
+    return fail(EPI_ERR_ARG, "epi_citizen_states_text_noalloc handles at most " + std::to_string(kNoAllocMaxAgents) +
+                                 " agents (its scratch array is fixed); this simulation has " + std::to_string(D.n));
+  const uint64_t need = static_cast<uint64_t>(D.n) * max_record_bytes(D.n, D.stride, D.height);
+  if (cap < need)
+    return fail(EPI_ERR_ARG, "d_text holds " + std::to_string(cap) + " bytes; the records may need " +
+                                 std::to_string(need));
+  unsigned long long* scratch = nullptr;
+  HIP_TRY(noalloc_scratch(&scratch));
+  int rc = settle(sim);  // the home layout holds the current state
+  if (rc != EPI_OK) return rc;
+  rc = fetch_states(sim);
+  if (rc != EPI_OK) return rc;
+  unsigned long long* len = reinterpret_cast<unsigned long long*>(d_len);
+  if (D.n == 0) {
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(len), 2, 0u, sim->stream));
+  } else {
+    const StreamArgs a{sim->v.pos + D.agent_off, sim->v.st + D.agent_off, D.n, sim->v.state,
+                       sim->state_h[0].hour, scratch, len, d_text};
+    HIP_TRY(launch_citizen_states(a, sim->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(sim->stream));
+  return EPI_OK;
 }
 
 int epi_citizen_states(epi_sim* sim, uint32_t index, epi_citizen_state* out, uint32_t cap, uint32_t* n,

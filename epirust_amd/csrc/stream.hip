@@ -16,12 +16,24 @@
 // that did not step the requested hour writes a total of 0.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "stream.hpp"
 
 namespace epi {
+
+// Scratch of the no-allocation formatter (epi_citizen_states_text_noalloc): the per-block
+// offsets, one entry per 256-agent block, for at most kNoAllocMaxAgents agents (512 KiB on
+// each GPU the library runs on, held until the process ends). Host code never passes this
+// name to a kernel: it is a host-side stand-in, not a device address. noalloc_scratch()
+// resolves the device's own copy once per device with hipGetSymbolAddress, and that
+// address is what the kernels get.
+static __device__ __align__(256) unsigned long long scratch[kNoAllocMaxBlocks];
+
 namespace {
 
 constexpr uint32_t kStreamBlock = 256;
+static_assert(kNoAllocMaxBlocks * kStreamBlock == kNoAllocMaxAgents, "no-allocation scratch size");
 
 // {"citizen_id":"  ","state":"  ?  ","location":{"x":  ,"y":  }}
 constexpr char kP0[] = "{\"citizen_id\":\"";
@@ -159,6 +171,29 @@ uint32_t max_record_bytes(uint32_t n, uint32_t stride, uint32_t height) {
 }
 
 uint32_t stream_blocks(uint32_t n) { return (n + kStreamBlock - 1) / kStreamBlock; }
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(unsigned long long** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {  // the current device's copy of the array, resolved once
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch[dev] = static_cast<unsigned long long*>(p);
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
 
 hipError_t launch_citizen_states(const StreamArgs& a, hipStream_t stream) {
   const uint32_t blocks = stream_blocks(a.n);

@@ -22,6 +22,15 @@ struct StreamArgs {
   char* text;                     // >= n * max_record_bytes(...) bytes
 };
 
+// Agents the no-allocation formatter handles: its per-block offsets live in a static
+// device array of kNoAllocMaxBlocks entries (8 bytes each).
+constexpr uint32_t kNoAllocMaxBlocks = 65536;
+constexpr uint32_t kNoAllocMaxAgents = kNoAllocMaxBlocks * 256u;
+
+// The current device's copy of the no-allocation scratch array (resolved once per device
+// with hipGetSymbolAddress; call with the device current).
+hipError_t noalloc_scratch(unsigned long long** out);
+
 // Upper bound of one record's length for `n` agents on a grid of the given extent.
 uint32_t max_record_bytes(uint32_t n, uint32_t stride, uint32_t height);
 uint32_t stream_blocks(uint32_t n);

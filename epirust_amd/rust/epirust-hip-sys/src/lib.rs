@@ -90,6 +90,7 @@ extern "C" {
     pub fn epi_citizen_states(sim: *mut EpiSim, index: u32, out: *mut EpiCitizenState, cap: u32, n: *mut u32,
                               hour: *mut u32) -> c_int;
     pub fn epi_stream_citizen_states(sim: *mut EpiSim, path: *const c_char) -> c_int;
+    pub fn epi_citizen_states_text_noalloc(sim: *mut EpiSim, d_text: *mut c_char, cap: u64, d_len: *mut u64) -> c_int;
     pub fn epi_current_counts(sim: *mut EpiSim, index: u32, out: *mut EpiCounts) -> c_int;
     pub fn epi_events(sim: *mut EpiSim, index: u32, out: *mut EpiEvent, cap: u32, n: *mut u32) -> c_int;
     pub fn epi_info(sim: *mut EpiSim, index: u32, n_sims: *mut u32, n_agents: *mut u32, grid_size: *mut u32,

@@ -119,6 +119,20 @@ int epi_citizen_states(epi_sim* sim, uint32_t index, epi_citizen_state* out, uin
  * closes it itself when the config sets enable_citizen_state_messages. */
 int epi_stream_citizen_states(epi_sim* sim, const char* path);
 
+/* The citizen-state message body of the handle's simulation after its last stepped hour,
+ * formatted on the device into caller-owned device memory: d_text receives the records
+ * joined by commas, exactly the text between '[' and ']' of the {"hr":h,"citizen_states":
+ * [...]} line that epi_stream_citizen_states writes for that hour, and d_len (8 device
+ * bytes) its length. No counterpart in the reference: a variant of the stream's device
+ * formatter for callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). It calls no allocating routine: the formatter's
+ * per-block offsets live in one static device array per GPU (512 KiB, held until the
+ * process ends), so it handles at most 16,777,216 agents, and calls on one GPU that
+ * overlap race on that array. cap must hold the records' upper bound, agents x the
+ * longest record (EPI_ERR_ARG otherwise, before anything is launched). Single-simulation
+ * handles only; returns once the text is written. */
+int epi_citizen_states_text_noalloc(epi_sim* sim, char* d_text, uint64_t cap, uint64_t* d_len);
+
 /* Counts of the last completed hour (hour 0: counts_at_start, engine/src/utils/util.rs:45-51). */
 int epi_current_counts(epi_sim* sim, uint32_t index, epi_counts* out);
 
