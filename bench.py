@@ -327,8 +327,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-extra", action="store_true", help="only the headline workload")
-    ap.add_argument("--surrogate-train", type=int, default=2048, help="training simulations of the surrogate leg")
-    ap.add_argument("--surrogate-eval", type=int, default=64, help="held-out points of the surrogate leg")
+    ap.add_argument("--surrogate-train", type=int, default=6144, help="training simulations of the surrogate leg")
+    ap.add_argument("--surrogate-eval", type=int, default=96, help="held-out points of the surrogate leg")
     ap.add_argument("--skip-surrogate", action="store_true")
     ap.add_argument("--profile-hours", type=int, default=0,
                     help="profiling driver for rocprofv3: run this many hours as separate kernel dispatches "

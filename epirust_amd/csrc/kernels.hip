@@ -131,10 +131,37 @@ __device__ __forceinline__ uint8_t grid_byte(const SimDesc& D, uint32_t w) {
   return static_cast<uint8_t>(1u + (infectious ? window_of(D, w) : 0u));
 }
 
-// Index of cell (x, y) in the padded grid / claim arrays of one simulation.
-__device__ __forceinline__ uint64_t cell_index(const SimDesc& D, int32_t x, int32_t y) {
-  return static_cast<uint64_t>(static_cast<uint32_t>(y) + kPadY) * D.pitch + static_cast<uint32_t>(x) + kPadX;
+// Index of cell (x, y) in the padded grid / claim arrays of one simulation. A simulation
+// has fewer than 2^32 padded cells (grid_size <= 32,000), so the index is 32-bit and a
+// load adds it to the simulation's base as an unsigned offset.
+__device__ __forceinline__ uint32_t cell_index(const SimDesc& D, int32_t x, int32_t y) {
+  return (static_cast<uint32_t>(y) + kPadY) * D.pitch + static_cast<uint32_t>(x) + kPadX;
 }
+
+// 3 x 3 neighbourhood masks: bit 3 * (dy + 1) + (dx + 1) stands for the cell at offset
+// (dx, dy), row-major, so bit 4 is the centre. moore8 drops the centre, which leaves the
+// Moore order of point.rs:57-61: (-1,-1) (0,-1) (1,-1) (-1,0) (1,0) (-1,1) (0,1) (1,1).
+__device__ __forceinline__ uint32_t moore8(uint32_t m9) { return (m9 & 0xFu) | ((m9 >> 1) & 0xF0u); }
+
+// Bit j (j = 0..2) set iff lo <= base + j <= hi.
+__device__ __forceinline__ uint32_t span3(int32_t base, int32_t lo, int32_t hi) {
+  const int32_t from = min(max(lo - base, 0), 3), past = min(max(base + 2 - hi, 0), 3);
+  return (0x7u << from) & (0x7u >> past) & 0x7u;
+}
+
+// Three grid bytes as 24 bits (columns px-1, px, px+1 of a window row).
+__device__ __forceinline__ uint32_t three(uint64_t row, int32_t byte_off) {
+  return static_cast<uint32_t>(row >> (8 * byte_off)) & 0xFFFFFFu;
+}
+// Per-byte tests of three bytes, each to bits 0..2. Exact: adding 0x7F to a byte's low
+// seven bits never carries into the next byte.
+__device__ __forceinline__ uint32_t pack3(uint32_t top_bits) {  // bits 7, 15, 23 -> 0, 1, 2
+  return ((top_bits >> 7) & 1u) | ((top_bits >> 14) & 2u) | ((top_bits >> 21) & 4u);
+}
+__device__ __forceinline__ uint32_t nonzero3(uint32_t v) {
+  return pack3((((v & 0x7F7F7Fu) + 0x7F7F7Fu) | v) & 0x808080u);
+}
+__device__ __forceinline__ uint32_t zero3(uint32_t v) { return nonzero3(v) ^ 0x7u; }
 
 // Claim key of agent `id` at `hour`: hours of day 7..23, 0 (the hours that move anyone)
 // rank 0..17 since the reset at hour of day 2; a later hour takes a smaller top field, so
@@ -245,6 +272,14 @@ struct Agent {
   __device__ __forceinline__ bool vacant(int32_t cx, int32_t cy) const {
     return occ[cell_index(D, cx, cy)] == 0;
   }
+  // 3 x 3 mask of the cells around (cx, cy) that lie in rect a and in the grid
+  // (area.rs:83-88 and allocation_map.rs:156-159 for each neighbour).
+  __device__ __forceinline__ uint32_t allowed9(const Rect& a, int32_t cx, int32_t cy) const {
+    const int32_t last = static_cast<int32_t>(D.g) - 1;
+    const uint32_t cols = span3(cx - 1, max(a.x0, 0), min(a.x1, last));
+    const uint32_t rows = span3(cy - 1, max(a.y0, 0), min(a.y1, last));
+    return cols * ((rows & 1u) | ((rows & 2u) << 2) | ((rows & 4u) << 4));  // cols copied to each row
+  }
   __device__ __forceinline__ bool in_grid(int32_t cx, int32_t cy) const {
     return cx >= 0 && cy >= 0 && cx < static_cast<int32_t>(D.g) && cy < static_cast<int32_t>(D.g);
   }
@@ -252,8 +287,7 @@ struct Agent {
   __device__ __forceinline__ void load_window(int32_t cx, int32_t cy) {
     const uint32_t px = static_cast<uint32_t>(cx) + kPadX;
     const uint32_t c0 = (px - 2u) & ~3u;
-    const uint32_t* p = reinterpret_cast<const uint32_t*>(
-        occ + static_cast<uint64_t>(static_cast<uint32_t>(cy) + kPadY - 2u) * D.pitch + c0);
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(occ + ((static_cast<uint32_t>(cy) + kPadY - 2u) * D.pitch + c0));
     const uint32_t q = D.pitch >> 2;
     uint32_t lo[5], hi[5];
 #pragma unroll
@@ -329,15 +363,12 @@ struct Agent {
       if (wbyte<0>(0) == 0) { tx = lx; ty = ly; }
       return;
     }
-    uint32_t cand = 0, n = 0;
-    for_moore([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      const int32_t cx = lx + moore_dx(k), cy = ly + moore_dy(k);
-      if (contains(a, cx, cy) && in_grid(cx, cy) && wbyte<moore_dy(k)>(moore_dx(k)) == 0) {
-        cand |= 1u << k;
-        ++n;
-      }
-    });
+    // vacant Moore neighbours of (lx, ly) in the area and the grid: the window rows
+    // ly-1 .. ly+1, bytes woff-1 .. woff+1, tested as three bytes at a time
+    const uint32_t vac9 = zero3(three(r1, woff - 1)) | (zero3(three(r2, woff - 1)) << 3) |
+                          (zero3(three(r3, woff - 1)) << 6);
+    uint32_t cand = moore8(vac9 & allowed9(a, lx, ly));
+    const uint32_t n = __builtin_popcount(cand);
     if (n == 0) return;
     uint32_t pick = below(move_words().w[2], n);
     for (uint32_t j = 0; j < pick; ++j) cand &= cand - 1;  // index of the pick-th set bit
@@ -361,17 +392,15 @@ struct Agent {
     const uint64_t top = (r0 & mm) | (r1 & m0) | (r2 & mp);
     const uint64_t mid = (r1 & mm) | (r2 & m0) | (r3 & mp);
     const uint64_t bot = (r2 & mm) | (r3 & m0) | (r4 & mp);
-    uint32_t hot = 0, high = 0;
-    for_moore([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      const int32_t cx = tx + moore_dx(k), cy = ty + moore_dy(k);
-      const uint64_t r = moore_dy(k) < 0 ? top : (moore_dy(k) == 0 ? mid : bot);
-      const uint32_t b = static_cast<uint32_t>(r >> (8 * (woff + ox + moore_dx(k)))) & 0xFFu;
-      if (contains(a, cx, cy) && in_grid(cx, cy) && b >= 2) {
-        hot |= 1u << k;
-        if (b == 3) high |= 1u << k;
-      }
-    });
+    // infectious neighbours (grid byte >= 2, i.e. a byte with a bit above bit 0) and the
+    // high-window ones (byte 3) among the cells of the area and the grid
+    const int32_t c = woff + ox - 1;
+    const uint32_t bt = three(top, c), bm = three(mid, c), bb = three(bot, c);
+    const uint32_t ok = allowed9(a, tx, ty);
+    uint32_t hot = moore8((nonzero3(bt & 0xFEFEFEu) | (nonzero3(bm & 0xFEFEFEu) << 3) |
+                           (nonzero3(bb & 0xFEFEFEu) << 6)) & ok);
+    const uint32_t high = moore8((nonzero3(bt & (bt >> 1) & 0x010101u) | (nonzero3(bm & (bm >> 1) & 0x010101u) << 3) |
+                                  (nonzero3(bb & (bb >> 1) & 0x010101u) << 6)) & ok);
     int blk = -1;
     uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0;
     while (hot) {
@@ -984,7 +1013,7 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
     loaded_together(w, p, t);
     if (w & (F_MOVE | F_DIRTY)) {
       uint8_t* occ = v.occ + D.cell_off;
-      const uint64_t old_ci = cell_index(D, static_cast<int32_t>(p & 0xFFFFu), static_cast<int32_t>(p >> 16));
+      const uint32_t old_ci = cell_index(D, static_cast<int32_t>(p & 0xFFFFu), static_cast<int32_t>(p >> 16));
       const uint8_t byte = grid_byte(D, w);
       bool moved = false;
       if (w & F_MOVE) {
@@ -994,8 +1023,8 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
         const bool house = (t & kTgtHouse) != 0u;
         const bool won = house ? (m1 != t && m2 != t && m3 != t) : (t & kTgtWon) != 0u;
         t &= kTgtCell;
-        const uint64_t ci = cell_index(D, static_cast<int32_t>(t & 0xFFFFu), static_cast<int32_t>(t >> 16));
-        if (won || (!house && v.claim[D.cell_off + ci] == claim_key(h, local))) {
+        const uint32_t ci = cell_index(D, static_cast<int32_t>(t & 0xFFFFu), static_cast<int32_t>(t >> 16));
+        if (won || (!house && (v.claim + D.cell_off)[ci] == claim_key(h, local))) {
           occ[old_ci] = 0;
           occ[ci] = byte;
           v.pos[gi] = t;

@@ -27,15 +27,17 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBM = 64, kBN = 64, kBK = 32;
-constexpr int kLdsK = kBK + 8;  // LDS row: 32 k of one m (or n) plus 8 of padding, 80 bytes
+constexpr int kLdsK = kBK + 8;  // LDS row: kBK k of one m (or n) plus 8 of padding
+constexpr int kChunks = kBK / 8;  // 16-byte vectors per tile row along k
+constexpr int kVecs = kBM * kBK / (256 * 8);  // 16-byte vectors per thread per operand tile
 
-// How one 64 x 32 operand tile (rows r = m or n, columns k) is read from global memory.
+// How one 64 x kBK operand tile (rows r = m or n, columns k) is read from global memory.
 enum : int { TILE_K_CONTIG = 0, TILE_R_CONTIG = 1, TILE_ELEMENT = 2 };
 
 // Element (r, k) of an operand is X[r * ld_r + k * ld_k]. A whole tile inside the matrix
-// whose contiguous dimension is 16-byte aligned is read as 16-byte vectors (8 bf16 per
-// thread, 256 threads): along k (ld_k == 1) or along r (ld_r == 1). Otherwise, at ragged
-// edges or odd strides, element by element with bounds checks.
+// whose contiguous dimension is 16-byte aligned is read as 16-byte vectors (kVecs of 8
+// bf16 per thread, 256 threads): along k (ld_k == 1) or along r (ld_r == 1). Otherwise, at
+// ragged edges or odd strides, element by element with bounds checks.
 __device__ __forceinline__ int tile_mode(const uint16_t* X, int64_t ld_r, int64_t ld_k, int r0, int k0, int R,
                                          int K) {
   const bool full = r0 + 64 <= R && k0 + kBK <= K;
@@ -45,33 +47,55 @@ __device__ __forceinline__ int tile_mode(const uint16_t* X, int64_t ld_r, int64_
   return TILE_ELEMENT;
 }
 
-__device__ __forceinline__ u16x8 load_tile(const uint16_t* X, int64_t ld_r, int64_t ld_k, int r0, int k0, int R,
-                                           int K, int mode, int tid) {
-  u16x8 v;
-  if (mode == TILE_K_CONTIG) {  // thread: row tid / 4, k (tid % 4) * 8 .. +7
-    v = *reinterpret_cast<const u16x8*>(X + static_cast<int64_t>(r0 + (tid >> 2)) * ld_r + k0 + (tid & 3) * 8);
-  } else if (mode == TILE_R_CONTIG) {  // thread: k tid / 8, rows (tid % 8) * 8 .. +7
-    v = *reinterpret_cast<const u16x8*>(X + static_cast<int64_t>(k0 + (tid >> 3)) * ld_k + r0 + (tid & 7) * 8);
-  } else {  // the TILE_K_CONTIG assignment, element by element
-    const int r = r0 + (tid >> 2), kb = k0 + (tid & 3) * 8;
+// Vector e (e = tid + 256 i, i < kVecs) of a tile: along k, row e / kChunks with k
+// (e % kChunks) * 8 .. +7; along r, k e / 8 with rows (e % 8) * 8 .. +7.
+struct TileRegs {
+  u16x8 v[kVecs];
+};
+
+__device__ __forceinline__ void load_tile(TileRegs& t, const uint16_t* X, int64_t ld_r, int64_t ld_k, int r0,
+                                          int k0, int R, int K, int mode, int tid) {
+  if (mode == TILE_K_CONTIG) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int k = kb + q;
-      v[q] = (r < R && k < K) ? X[static_cast<int64_t>(r) * ld_r + static_cast<int64_t>(k) * ld_k] : 0;
+    for (int i = 0; i < kVecs; ++i) {
+      const int e = tid + 256 * i;
+      t.v[i] = *reinterpret_cast<const u16x8*>(X + static_cast<int64_t>(r0 + e / kChunks) * ld_r + k0 + (e % kChunks) * 8);
+    }
+  } else if (mode == TILE_R_CONTIG) {
+#pragma unroll
+    for (int i = 0; i < kVecs; ++i) {
+      const int e = tid + 256 * i;
+      t.v[i] = *reinterpret_cast<const u16x8*>(X + static_cast<int64_t>(k0 + (e >> 3)) * ld_k + r0 + (e & 7) * 8);
+    }
+  } else {  // ragged edges and odd strides: the along-k assignment, element by element
+#pragma unroll 1
+    for (int i = 0; i < kVecs; ++i) {
+      const int e = tid + 256 * i;
+      const int r = r0 + e / kChunks, kb = k0 + (e % kChunks) * 8;
+      u16x8 v;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = kb + q;
+        v[q] = (r < R && k < K) ? X[static_cast<int64_t>(r) * ld_r + static_cast<int64_t>(k) * ld_k] : 0;
+      }
+      t.v[i] = v;
     }
   }
-  return v;
 }
 
-// LDS tile s[r][k] (k contiguous): one 16-byte store, or eight 2-byte stores for a tile
-// read along r.
-__device__ __forceinline__ void store_tile(uint16_t (*s)[kLdsK], const u16x8& v, int mode, int tid) {
-  if (mode == TILE_R_CONTIG) {
-    const int k = tid >> 3, rb = (tid & 7) * 8;
+// LDS tile s[r][k] (k contiguous): 16-byte stores, or 2-byte stores for a tile read
+// along r.
+__device__ __forceinline__ void store_tile(uint16_t (*s)[kLdsK], const TileRegs& t, int mode, int tid) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) s[rb + q][k] = v[q];
-  } else {
-    *reinterpret_cast<u16x8*>(&s[tid >> 2][(tid & 3) * 8]) = v;
+  for (int i = 0; i < kVecs; ++i) {
+    const int e = tid + 256 * i;
+    if (mode == TILE_R_CONTIG) {
+      const int k = e >> 3, rb = (e & 7) * 8;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) s[rb + q][k] = t.v[i][q];
+    } else {
+      *reinterpret_cast<u16x8*>(&s[e / kChunks][(e % kChunks) * 8]) = t.v[i];
+    }
   }
 }
 
@@ -98,7 +122,7 @@ __device__ __forceinline__ float gelu_grad(float x) {
 }
 
 // C tile 64 x 64 per block of four wave64s (2 x 2), each a 32 x 32 sub-tile accumulated
-// with v_mfma_f32_32x32x16_bf16 (gfx950): per 32-wide k step, two MFMAs whose A operand is
+// with v_mfma_f32_32x32x16_bf16 (gfx950): per kBK-wide k step, kBK / 16 MFMAs whose A operand is
 // row (lane % 32) with k 8 * (lane / 32) .. +7, read from LDS as one 16-byte vector, and B
 // likewise by column. The next k tile's global loads are issued before the current one's
 // MFMAs (register prefetch), into the other of two LDS buffers, so each k step has one
@@ -114,8 +138,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 
   int ma = tile_mode(g.A, g.lda_m, g.lda_k, m0, 0, g.M, g.K);
   int mb = tile_mode(g.B, g.ldb_n, g.ldb_k, n0, 0, g.N, g.K);
-  u16x8 va = load_tile(g.A, g.lda_m, g.lda_k, m0, 0, g.M, g.K, ma, tid);
-  u16x8 vb = load_tile(g.B, g.ldb_n, g.ldb_k, n0, 0, g.N, g.K, mb, tid);
+  TileRegs va, vb;
+  load_tile(va, g.A, g.lda_m, g.lda_k, m0, 0, g.M, g.K, ma, tid);
+  load_tile(vb, g.B, g.ldb_n, g.ldb_k, n0, 0, g.N, g.K, mb, tid);
   int buf = 0;
   for (int k0 = 0; k0 < g.K; k0 += kBK) {
     store_tile(sA[buf], va, ma, tid);
@@ -125,8 +150,8 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     if (k1 < g.K) {  // uniform: prefetch the next tile while this one is multiplied
       ma = tile_mode(g.A, g.lda_m, g.lda_k, m0, k1, g.M, g.K);
       mb = tile_mode(g.B, g.ldb_n, g.ldb_k, n0, k1, g.N, g.K);
-      va = load_tile(g.A, g.lda_m, g.lda_k, m0, k1, g.M, g.K, ma, tid);
-      vb = load_tile(g.B, g.ldb_n, g.ldb_k, n0, k1, g.N, g.K, mb, tid);
+      load_tile(va, g.A, g.lda_m, g.lda_k, m0, k1, g.M, g.K, ma, tid);
+      load_tile(vb, g.B, g.ldb_n, g.ldb_k, n0, k1, g.N, g.K, mb, tid);
     }
 #pragma unroll
     for (int kk = 0; kk < kBK; kk += 16) {
