@@ -9,6 +9,11 @@ is a thin ctypes mirror of the reference's standalone entry points:
 
 There is no CPU fallback: if the library is missing or no gfx950 device is present the
 calls raise.
+
+The disease model is compiled into the library (the reference's DiseaseHandler type
+parameter, engine/src/state_machine/disease_handler.rs:27-44): EPIRUST_GPU_LIB names a
+library built with another handler (python -m epirust_amd.build --disease-handler
+HEADER:TYPE --out PATH) to load instead of libepigpu.so.
 """
 import ctypes
 import json
@@ -19,7 +24,7 @@ import numpy as np
 __all__ = ["Simulation", "Batch", "EngineApp", "EpiError", "lib", "LIB_PATH"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libepigpu.so")
+LIB_PATH = os.environ.get("EPIRUST_GPU_LIB") or os.path.join(PKG, "libepigpu.so")
 EXE_PATH = os.path.join(PKG, "epirust-gpu")
 
 EPI_FLAG_DETERMINISTIC = 1
@@ -113,7 +118,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise EpiError(-1, "libepigpu.so is not built (python -m epirust_amd.build)")
+            raise EpiError(-1, "%s is not built (python -m epirust_amd.build)" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SYMBOLS.items():
             fn = getattr(L, name)

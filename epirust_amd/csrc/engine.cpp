@@ -21,6 +21,7 @@
 
 #include "epirust_gpu.h"
 #include "config.hpp"
+#include "disease.hpp"
 #include "init.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
@@ -45,23 +46,6 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 bool valid_p(double p) { return p >= 0.0 && p <= 1.0; }
-
-// is_to_be_hospitalized per transmission window (common/src/disease/mod.rs:97-99)
-uint32_t hospital_window_bits(const Config& c) {
-  uint32_t bits = 0;
-  if (0.0 >= c.high_transmission_rate) bits |= 1u;
-  if (c.regular_transmission_rate >= c.high_transmission_rate) bits |= 2u;
-  if (c.high_transmission_rate >= c.high_transmission_rate) bits |= 4u;
-  return bits;
-}
-
-uint8_t host_grid_byte(const Config& c, uint32_t w) {
-  if (st_state(w) != ST_I || (w & F_HOSP)) return 1;
-  const uint32_t d = static_cast<uint32_t>(static_cast<int32_t>(st_day(w)) + st_imm(w));
-  if (c.regular_transmission_start_day < d && d <= c.high_transmission_start_day) return 2;
-  if (c.high_transmission_start_day < d && d <= c.last_day) return 3;
-  return 1;
-}
 
 Rect building_rect(int32_t sx, int32_t sy, int32_t nx, int32_t size, uint32_t k) {
   int32_t x0 = sx + static_cast<int32_t>(k % static_cast<uint32_t>(nx)) * size;
@@ -655,17 +639,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     D.per_office = (D.n + D.n_offices - 1u) / D.n_offices;
     D.n_slots = D.n_offices * D.per_office;
     D.fd_per_office = make_fastdiv(std::max(D.per_office, 1u));
-    D.regular_start = c.regular_transmission_start_day;
-    D.high_start = c.high_transmission_start_day;
-    D.last_day = c.last_day;
-    D.exposed_duration = c.exposed_duration;
-    D.pre_symptomatic_duration = c.pre_symptomatic_duration;
-    D.thr_regular = bernoulli_threshold(c.regular_transmission_rate);
-    D.thr_high = bernoulli_threshold(c.high_transmission_rate);
-    D.thr_death = bernoulli_threshold(c.death_rate);
-    D.thr_symptoms = bernoulli_threshold(1.0 - c.percentage_asymptomatic_population);
-    D.thr_severe = bernoulli_threshold(c.percentage_severe_infected_population);
-    D.hosp_window = hospital_window_bits(c);
+    D.dz = Disease::params(c);
     D.has_lockdown = c.has_lockdown;
     D.lockdown_at = c.lockdown_at_number_of_infections;
     D.has_hospital = c.has_build_new_hospital;
@@ -795,7 +769,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     for (uint32_t a = 0; a < D.n; ++a) {
       const uint32_t p = P.pos[a];
       grid[static_cast<size_t>((p >> 16) + kPadY) * D.pitch + (p & 0xFFFFu) + kPadX] =
-          host_grid_byte(s->cfg[i], P.st[a]);
+          occupant_byte(D.dz, P.st[a]);
     }
     HIP_TRY(hipMemcpy(v.occ + D.cell_off, grid.data(), cells, hipMemcpyHostToDevice));
   }
@@ -1120,7 +1094,7 @@ int epi_read_agents(epi_sim* sim, uint32_t index, epi_agents_soa* dst) {
     if (dst->state) dst->state[i] = static_cast<uint8_t>(state);
     if (dst->severity) dst->severity[i] = static_cast<uint8_t>(state == ST_I ? sev : 0);
     if (dst->infection_day) dst->infection_day[i] = state == ST_I ? st_day(w) : 0;
-    if (dst->at_hour) dst->at_hour[i] = (state == ST_E || (state == ST_I && sev == SEV_PRE)) ? tm[i] : 0;
+    if (dst->at_hour) dst->at_hour[i] = Disease::timed(w) ? tm[i] : 0;
     if (dst->flags)
       dst->flags[i] = static_cast<uint8_t>(((w & F_HOSP) ? 1 : 0) | ((w & F_VACC) ? 2 : 0) | ((w & F_ISO) ? 4 : 0) |
                                            ((w & F_WQ) ? 8 : 0) | ((w & F_PT) ? 16 : 0));

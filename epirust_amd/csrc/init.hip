@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "disease.hpp"
 #include "init.hpp"
 #include "philox.hpp"
 
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(kInitBlock) void k_init_cells(InitArgs a, HouseLayo
 struct GridArgs {
   uint8_t* occ;
   uint32_t pitch;
-  uint32_t regular_start, high_start, last_day;
+  DiseaseParams dz;
 };
 
 // The agent's grid byte: 1 occupied, 2 / 3 infectious in the regular / high window.
@@ -239,13 +240,7 @@ __global__ __launch_bounds__(kInitBlock) void k_init_grid(InitArgs a, GridArgs g
   const uint32_t i = blockIdx.x * kInitBlock + threadIdx.x;
   if (i >= a.n) return;
   const uint32_t p = a.pos[i], w = a.st[i];
-  uint8_t byte = 1;
-  if (((w >> B_STATE) & M_STATE) == ST_I && !(w & F_HOSP)) {
-    const uint32_t d = ((w >> B_DAY) & M_DAY) + ((w >> B_IMM) & M_IMM) - 2u;  // u32 wrap like the reference
-    if (g.regular_start < d && d <= g.high_start) byte = 2;
-    else if (g.high_start < d && d <= g.last_day) byte = 3;
-  }
-  g.occ[static_cast<size_t>((p >> 16) + kPadY) * g.pitch + (p & 0xFFFFu) + kPadX] = byte;
+  g.occ[static_cast<size_t>((p >> 16) + kPadY) * g.pitch + (p & 0xFFFFu) + kPadX] = occupant_byte(g.dz, w);
 }
 
 void set_digit(Composite& c, uint32_t shift, uint32_t d) {
@@ -381,8 +376,7 @@ hipError_t synthesize_on_device(const Config& cfg, uint64_t seed, const Geometry
                        L);
     INIT_TRY("k_init_cells", hipGetLastError());
   }
-  const GridArgs g{t.occ, t.pitch, cfg.regular_transmission_start_day, cfg.high_transmission_start_day,
-                   cfg.last_day};
+  const GridArgs g{t.occ, t.pitch, Disease::params(cfg)};
   hipLaunchKernelGGL(k_init_grid, dim3(blocks), dim3(kInitBlock), 0, stream, a, g);
   INIT_TRY("k_init_grid", hipGetLastError());
   // the temporaries are freed on return: finish with them first

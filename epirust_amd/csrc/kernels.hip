@@ -35,6 +35,7 @@
 // integer work bound by HBM traffic.
 #include <hip/hip_runtime.h>
 
+#include "disease.hpp"
 #include "kernels.hpp"
 #include "philox.hpp"
 
@@ -115,22 +116,6 @@ __device__ __forceinline__ uint32_t set_fld(uint32_t w, uint32_t b, uint32_t m, 
   return (w & ~(m << b)) | ((v & m) << b);
 }
 
-// Transmission window of (infection_day + immunity) as u32 (citizen/mod.rs:182-185,
-// common/src/disease/mod.rs:88-95): 1 regular, 2 high, 0 rate 0.0.
-__device__ __forceinline__ uint32_t window_of(const SimDesc& D, uint32_t w) {
-  const uint32_t day = fld(w, B_STATE, M_STATE) == ST_I ? fld(w, B_DAY, M_DAY) : 0u;
-  const uint32_t d = static_cast<uint32_t>(static_cast<int32_t>(day) + static_cast<int32_t>(fld(w, B_IMM, M_IMM)) - 2);
-  if (D.regular_start < d && d <= D.high_start) return 1u;
-  if (D.high_start < d && d <= D.last_day) return 2u;
-  return 0u;
-}
-
-// Grid byte of an occupied cell: 1 + (infectious window or 0).
-__device__ __forceinline__ uint8_t grid_byte(const SimDesc& D, uint32_t w) {
-  const bool infectious = fld(w, B_STATE, M_STATE) == ST_I && !(w & F_HOSP);
-  return static_cast<uint8_t>(1u + (infectious ? window_of(D, w) : 0u));
-}
-
 // Index of cell (x, y) in the padded grid / claim arrays of one simulation. A simulation
 // has fewer than 2^32 padded cells (grid_size <= 32,000), so the index is 32-bit and a
 // load adds it to the simulation's base as an unsigned offset.
@@ -198,6 +183,7 @@ struct Agent {
   // fields would otherwise become a per-lane load of the chosen field.
   const uint64_t thr_regular, thr_high;
   uint32_t id, h, w, tm, wsa;
+  uint32_t tm0 = 0;  // tm as stored: loaded for the states that carry an at_hour, else 0
   const uint32_t* tm_arr = nullptr;  // this agent's stored tm word
   const uint32_t* claims = nullptr;  // this simulation's claim words
   // Claim word of the chosen cell as read right after a local move is chosen (~0u when not
@@ -222,7 +208,8 @@ struct Agent {
   uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;  // named, so no variable-index access exists
 
   __device__ Agent(const SimDesc& d, const SimState& s, const uint8_t* o)
-      : D(d), S(s), occ(o), thr_regular(d.thr_regular), thr_high(d.thr_high) {}
+      : D(d), S(s), occ(o), thr_regular(Disease::transmission_threshold(d.dz, 1u)),
+        thr_high(Disease::transmission_threshold(d.dz, 2u)) {}
 
   __device__ __forceinline__ u32x4 dr(uint32_t site, uint32_t block) const {
     return philox4x32_10(id, h, site, block, D.seed_lo, D.seed_hi);
@@ -240,14 +227,20 @@ struct Agent {
     return !(symptomatic() || (w & F_HOSP) || state() == ST_D || (w & F_ISO));
   }
   // Home house id % H and office id % O, tiled row-major (area.rs:95-117), once per hour
-  // with the division constants of the tiling.
-  __device__ __forceinline__ void locate_buildings() {
+  // with the division constants of the tiling. The office is needed only at hours of day
+  // 8..16: a worker's area is its office from 8 to 16 (goto home resets it at 16, the
+  // sleep hours reset it to home), the office is the target at 8, and the symptomatic-at-
+  // work override of goto home is tested at 16; at other hours ox, oy are left unset and
+  // every expression that reads them is masked by a condition that is false then.
+  __device__ __forceinline__ void locate_buildings(bool office) {
     const uint32_t hk = fast_mod(id, D.fd_houses), hrow = fast_div(hk, D.fd_house_nx);
     hx = D.house_sx + static_cast<int32_t>(hk - hrow * D.fd_house_nx.d) * 2;
     hy = D.house_sy + static_cast<int32_t>(hrow) * 2;
-    const uint32_t ok = fast_mod(id, D.fd_offices), orow = fast_div(ok, D.fd_office_nx);
-    ox = D.office_sx + static_cast<int32_t>(ok - orow * D.fd_office_nx.d) * 10;
-    oy = D.office_sy + static_cast<int32_t>(orow) * 10;
+    if (office) {
+      const uint32_t ok = fast_mod(id, D.fd_offices), orow = fast_div(ok, D.fd_office_nx);
+      ox = D.office_sx + static_cast<int32_t>(ok - orow * D.fd_office_nx.d) * 10;
+      oy = D.office_sy + static_cast<int32_t>(orow) * 10;
+    }
   }
   __device__ __forceinline__ Rect home() const { return Rect{hx, hy, hx + 1, hy + 1}; }
   __device__ __forceinline__ bool in_house(int32_t cx, int32_t cy) const {
@@ -346,8 +339,8 @@ struct Agent {
   // goto_area's gate for a working citizen: can_move, or the symptomatic-at-work override.
   __device__ __forceinline__ uint32_t tele_act(const Rect& target) const {
     const uint32_t s = ws();
-    const bool override_movement = (s == WS_NORMAL || s == WS_ESSENTIAL) && contains(work(), x, y) &&
-                                   same_rect(target, home()) && symptomatic();
+    const bool override_movement = (s == WS_NORMAL || s == WS_ESSENTIAL) && symptomatic() &&
+                                   same_rect(target, home()) && contains(work(), x, y);
     return (can_move() || override_movement) ? ACT_TELE : ACT_NONE;
   }
 
@@ -384,7 +377,7 @@ struct Agent {
   // bit masks, so no variable indexing), then one Philox block per pair of neighbours,
   // drawn in Moore order only for eligible ones.
   __device__ __forceinline__ bool on_susceptible() {
-    if ((w & F_WQ) || (w & F_VACC)) return false;
+    if (!Disease::can_catch(w)) return false;
     const Rect a = current_area();
     cover(tx, ty);
     // No infectious cell (byte >= 2) anywhere in the 5 x 5 window, a superset of the
@@ -422,32 +415,23 @@ struct Agent {
     return false;
   }
 
-  // disease_state_machine.rs:53-70 + default_disease_handler.rs:41-62
+  // The stored at_hour of a state that carries one (Exposed, Pre), read where it is needed.
+  __device__ __forceinline__ void load_tm() {
+    if (Disease::timed(w)) tm = tm0 = *tm_arr;
+  }
+
+  // DiseaseStateMachine::next (disease_state_machine.rs:53-70): the handler's transition of
+  // the current state; on_susceptible is the neighbour scan above.
   __device__ __forceinline__ void update_infection_dynamics() {
     const uint32_t s = state();
     if (s == ST_S) {
-      if (on_susceptible()) {
-        w = set_fld(w, B_STATE, M_STATE, ST_E);
-        tm = h;
-      }
+      if (on_susceptible()) Disease::on_caught(h, w, tm);
     } else if (s == ST_E) {
-      tm = *tm_arr;
-      const u32x4& d = disease_words();
-      const int32_t rf = static_cast<int32_t>(below(d.w[0], 3)) - 1;  // RANGE_FOR_EXPOSED
-      if (h - tm >= static_cast<uint32_t>(static_cast<int32_t>(D.exposed_duration) + rf)) {
-        const bool symptoms = bernoulli(D.thr_symptoms, join64(d.w[2], d.w[3]));
-        w = set_fld(w, B_STATE, M_STATE, ST_I);
-        w = set_fld(w, B_DAY, M_DAY, 0);
-        w = set_fld(w, B_SEV, M_SEV, symptoms ? SEV_PRE : SEV_ASYM);
-        tm = symptoms ? h : 0u;
-      }
-    } else if (s == ST_I && sev() == SEV_PRE) {
-      tm = *tm_arr;
-      if (h - tm >= D.pre_symptomatic_duration) {
-        const u32x4& d = disease_words();
-        w = set_fld(w, B_SEV, M_SEV, bernoulli(D.thr_severe, join64(d.w[2], d.w[3])) ? SEV_SEVERE : SEV_MILD);
-        tm = 0;
-      }
+      load_tm();
+      Disease::on_exposed(D.dz, h, disease_words(), w, tm);
+    } else if (s == ST_I) {
+      load_tm();
+      Disease::on_infected(D.dz, h, disease_words(), w, tm);
     }
   }
 
@@ -515,8 +499,7 @@ struct Agent {
 
   // citizen/mod.rs:351-365 with allocation_map.rs:144-150
   __device__ __forceinline__ void hospitalize() {
-    if ((w & F_HOSP) || state() != ST_I || sev() != SEV_SEVERE) return;
-    if (!((D.hosp_window >> window_of(D, w)) & 1u)) return;
+    if ((w & F_HOSP) || !Disease::is_to_be_hospitalize(D.dz, w)) return;
     const uint32_t fv = S.first_vacant;
     if (fv != ~0u) {
       const Rect hr = D.hosp[S.hosp_version];
@@ -532,24 +515,11 @@ struct Agent {
     }
   }
 
-  // citizen/mod.rs:397-413 + default_disease_handler.rs:88-102
+  // citizen/mod.rs:397-413: the handler's on_routine_end, then the recovered go home and the
+  // recovered or deceased leave hospital.
   __device__ __forceinline__ void deceased() {
-    if (state() == ST_I) {
-      const uint32_t day = fld(w, B_DAY, M_DAY), sv = sev();
-      uint32_t ns = ST_I;
-      if (sv == SEV_ASYM && day == 9u) ns = ST_R;        // ASYMPTOMATIC_LAST_DAY
-      else if (sv == SEV_MILD && day == 12u) ns = ST_R;  // MILD_INFECTED_LAST_DAY
-      else if (sv == SEV_SEVERE && day == D.last_day) {
-        const u32x4& d = disease_words();
-        ns = bernoulli(D.thr_death, join64(d.w[2], d.w[3])) ? ST_D : ST_R;
-      }
-      if (ns != ST_I) {
-        w = set_fld(w, B_STATE, M_STATE, ns);
-        w = set_fld(w, B_SEV, M_SEV, 0);
-        w = set_fld(w, B_DAY, M_DAY, 0);
-        tm = 0;
-      }
-    }
+    load_tm();
+    Disease::on_routine_end(D.dz, disease_words(), w, tm);
     if (state() == ST_R) {
       int32_t px, py;
       random_point(home(), px, py);
@@ -774,7 +744,7 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     Agent a(D, *S, occ);
     a.id = local;
     a.h = h;
-    a.locate_buildings();
+    a.locate_buildings(hd >= 8 && hd <= 16);
     if (hd != 0) a.mv = a.dr(SITE_MOVE, 0);
     loaded_together(w_stored, p, a.mv);
     a.w = sweep.mask ? apply_sweep(w_stored, sweep, local, D) : w_stored;
@@ -794,13 +764,8 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     const bool staff = a.ws() == WS_STAFF;
     a.wsa = staff ? v.wsa[gid] : 0u;
     const uint32_t wsa0 = a.wsa;
-    bool tm_dirty = false;
 
-    {
-      const uint32_t s0 = a.state(), v0 = a.sev();
-      if (hd >= 7 && (s0 == ST_E || (s0 == ST_I && (v0 == SEV_PRE || (hd == 23 && v0 == SEV_SEVERE)))))
-        a.dz = a.dr(SITE_DISEASE, 0);
-    }
+    if (Disease::draws_needed(a.w, hd)) a.dz = a.dr(SITE_DISEASE, 0);
     if (hd == 0) {
       if (a.state() == ST_I) {  // increment_infection_day (state.rs:69-73)
         const uint32_t day = fld(a.w, B_DAY, M_DAY);
@@ -808,19 +773,18 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
       }
       a.hospitalize();
     } else if (hd == 23) {
-      const uint32_t s0 = a.state();
       a.deceased();
-      tm_dirty = a.state() != s0;
     } else {
-      const uint32_t s0 = a.state(), sv0 = a.sev();
       a.perform_movements(hd);
-      tm_dirty = a.state() != s0 || a.sev() != sv0;
     }
     // Everything the stores need is computed first: a descriptor field read after a store
     // can no longer come from the scalar cache and costs a vector round trip.
     key = claim_key(h, local);
     moves = (a.tx != a.x || a.ty != a.y) && key < a.seen_claim;
-    if (grid_byte(D, a.w) != grid_byte(D, w0)) a.w |= F_DIRTY;
+    // the grid byte depends only on the state, the hospitalized flag, the infection day and
+    // the immunity, so it is recomputed only when one of those changed this hour
+    constexpr uint32_t kByteFields = (M_STATE << B_STATE) | F_HOSP | (M_DAY << B_DAY) | (M_IMM << B_IMM);
+    if (((a.w ^ w0) & kByteFields) && occupant_byte(D.dz, a.w) != occupant_byte(D.dz, w0)) a.w |= F_DIRTY;
     ci = D.cell_off + cell_index(D, a.tx, a.ty);
     const uint32_t s = a.state();
     cat = s == ST_S ? 0u : s == ST_E ? 1u : s == ST_I ? ((a.w & F_HOSP) ? 3u : 2u) : s == ST_R ? 4u : 5u;
@@ -844,7 +808,7 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     // the lowest-id claimant of such a cell is found among the agent's lower housemates,
     // whose target words the commit reads: no global claim is made.
     house_move = !v.office && moves && a.in_house(a.tx, a.ty);
-    if (tm_dirty) v.tm[gid] = a.tm;
+    if (a.tm != a.tm0) v.tm[gid] = a.tm;
     if (staff && a.wsa != wsa0) v.wsa[gid] = a.wsa;
   }
   // Block claim filter (office hours). The office layout puts each office's workers in one
@@ -1018,7 +982,7 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
     if (w & (F_MOVE | F_DIRTY)) {
       uint8_t* occ = v.occ + D.cell_off;
       const uint32_t old_ci = cell_index(D, static_cast<int32_t>(p & 0xFFFFu), static_cast<int32_t>(p >> 16));
-      const uint8_t byte = grid_byte(D, w);
+      const uint8_t byte = occupant_byte(D.dz, w);
       bool moved = false;
       if (w & F_MOVE) {
         // won: decided by the office-hour claim filter; a house move wins unless a lower

@@ -80,6 +80,17 @@ inline uint64_t padded_cells(uint32_t stride, uint32_t height) {
   return static_cast<uint64_t>(padded_pitch(stride)) * (height + 2u * kPadY);
 }
 
+// The constants of the reference's Disease (common/src/disease/mod.rs:26-44) in the form the
+// step uses: window bounds in days, durations in hours, rates as Bernoulli thresholds, and
+// bit c of hosp_window set when is_to_be_hospitalized holds in transmission window c. A
+// disease handler (disease.hpp) fills them from the config and is their only reader.
+struct DiseaseParams {
+  uint32_t regular_start, high_start, last_day;
+  uint32_t exposed_duration, pre_symptomatic_duration;
+  uint64_t thr_regular, thr_high, thr_death, thr_symptoms, thr_severe;
+  uint32_t hosp_window;
+};
+
 // Read-only per-simulation descriptor (device copy built once at create).
 struct SimDesc {
   uint32_t n;          // agents
@@ -100,11 +111,7 @@ struct SimDesc {
   int32_t office_sx, office_sy, office_nx;
   uint32_t n_offices;
   FastDiv fd_houses, fd_house_nx, fd_offices, fd_office_nx;  // id -> building index -> (column, row)
-  // disease (common/src/disease/mod.rs:26-44)
-  uint32_t regular_start, high_start, last_day;
-  uint32_t exposed_duration, pre_symptomatic_duration;
-  uint64_t thr_regular, thr_high, thr_death, thr_symptoms, thr_severe;
-  uint32_t hosp_window;  // bit w: is_to_be_hospitalized holds in transmission window w
+  DiseaseParams dz;  // the compiled-in disease handler's constants (disease.hpp)
   // interventions
   uint32_t has_lockdown, lockdown_at;
   uint32_t has_hospital, spread_rate_threshold;

@@ -39,6 +39,23 @@ def test_library_exports_every_declared_symbol():
     assert set(declared_functions()) <= set(E.SYMBOLS)
 
 
+def test_disease_handler_variant_exports_every_declared_symbol():
+    """The library built with the test disease handler (tests/handlers/recovering_disease.hpp)
+    is a drop-in for libepigpu.so: same entry points, loadable on its own."""
+    path = os.path.join(ROOT, "tests", "handlers", "libepigpu_recovering.so")
+    if not os.path.exists(path):
+        pytest.fail("%s is not built (python -c 'import __graft_entry__ as e; e.build()')" % path)
+    L = ctypes.CDLL(path)
+    for name in declared_functions():
+        assert hasattr(L, name), name
+
+
+def test_disease_handler_build_rejects_a_missing_header(tmp_path):
+    from epirust_amd import build as B
+    with pytest.raises(RuntimeError, match="disease handler header not found"):
+        B.build_handler_variant(str(tmp_path / "none.hpp"), "epi::None", str(tmp_path / "x.so"))
+
+
 def _create(text):
     h = ctypes.c_void_p()
     rc = E.lib().epi_create(text.encode(), 1, 1, ctypes.byref(h))
