@@ -114,16 +114,23 @@ struct epi_sim {
   bool any_empty = false;
   uint32_t flags = 0;
   LaunchShape shape;
-  std::vector<BlockRef> agent_blocks, office_blocks, hosp_blocks;
+  std::vector<AgentBlock> agent_blocks, office_blocks;
+  std::vector<BlockRef> hosp_blocks;
+  // per simulation: the number of Normal and Essential workers of each office, in office
+  // order (their slots come first in the office layout, office by office)
+  std::vector<std::vector<uint32_t>> office_workers;
   std::vector<uint32_t> vacc_hour;
   std::vector<unsigned long long> vacc_thr;
   DeviceBuffers mem;
   DevView v{};         // agent arrays in the home layout
   DevView v_office{};  // the same view over the office-layout copies (hours of day 8..15)
-  TileRef* d_tiles = nullptr;  // layout transpose tiles of every simulation
+  AgentBlock* d_agent_blocks = nullptr;
+  AgentBlock* d_office_blocks = nullptr;
+  uint32_t* d_office_agent = nullptr;  // office layout: slot -> agent id
+  uint32_t* d_office_slot = nullptr;   // office layout: agent id -> slot
+  TileRef* d_tiles = nullptr;          // layout move tiles of every simulation
   uint32_t n_tiles = 0;
-  BlockRef* d_agent_blocks = nullptr;
-  BlockRef* d_office_blocks = nullptr;
+  uint64_t cap_blocks = 0, cap_office_blocks = 0;  // entries of the two device tables
   // Day graphs run hours of day 3..6 as one epilogue-only launch: no simulation has a
   // vaccination at an hour of day 2..5, so no sweep is pending at 3..6.
   bool quiet_hours = false;
@@ -233,37 +240,121 @@ int build_launch_tables(epi_sim* s) {
       start[x] = acc;
       acc += nb / kXcds + (x < nb % kXcds ? 1u : 0u);
     }
-    for (uint32_t j = 0; j < nb; ++j)
-      s->agent_blocks.push_back(BlockRef{i, sorted[start[j % kXcds] + j / kXcds] * B});
+    for (uint32_t j = 0; j < nb; ++j) {
+      const uint32_t first = sorted[start[j % kXcds] + j / kXcds] * B;
+      s->agent_blocks.push_back(AgentBlock{i, first, std::min(B, n - first), 1, 0, 1, 0, 0});
+    }
   }
   s->shape.agent_blocks = static_cast<uint32_t>(s->agent_blocks.size());
-  // Office layout: floor(B / K) whole offices of K slots per block; n_slots is a multiple
-  // of K, so no office is split. Idle lanes cost latency hiding (at 256 threads and 139
-  // slots per office, 10M agents ran 14% slower), so this needs blocks at least 7/8 full;
-  // otherwise blocks of B consecutive slots. Either way a block has more than B / 2 slots,
-  // so there is at most one block per 32 slots.
+  // Office layout: the workers' blocks hold whole offices, as many consecutive ones as fit
+  // B slots and the claim filter's kMaxFilterOffices; an office of more than B workers
+  // spans blocks of its own. Then blocks of B slots over the other agents. Idle lanes cost
+  // latency hiding, and offices are packed in order, so a block is more than half full
+  // unless its next office alone would not fit beside it.
+  // The two kinds of blocks are interleaved evenly in dispatch order: the other agents'
+  // blocks make more global claims and fewer LDS ones, and run beside the workers' instead
+  // of all at the end of the launch.
   s->office_blocks.clear();
-  for (uint32_t i = 0; i < s->n_sims; ++i) {
+  std::vector<AgentBlock> wb, rb;
+  for (uint32_t i = 0; i < s->n_sims && !s->office_workers.empty(); ++i) {
     SimDesc& D = s->desc[i];
-    const uint32_t K = D.per_office;
-    const uint32_t whole = (K >= 1 && K <= B) ? (B / K) * K : 0u;
-    D.office_span = 8u * whole >= 7u * B ? whole : B;
-    const uint32_t nb = (D.n_slots + D.office_span - 1) / D.office_span;
-    D.office_n_blocks = nb;
+    const std::vector<uint32_t>& cnt = s->office_workers[i];
     D.office_block_begin = static_cast<uint32_t>(s->office_blocks.size());
-    for (uint32_t b = 0; b < nb; ++b) s->office_blocks.push_back(BlockRef{i, b * D.office_span});
+    wb.clear();
+    rb.clear();
+    uint32_t slot = 0, j = 0;
+    const uint32_t O = static_cast<uint32_t>(cnt.size());
+    while (j < O) {
+      if (cnt[j] > B) {
+        for (uint32_t k = 0; k < cnt[j]; k += B) wb.push_back(AgentBlock{i, slot + k, std::min(B, cnt[j] - k), j, j, 1, 0, 0});
+        slot += cnt[j];
+        ++j;
+        continue;
+      }
+      const uint32_t first = slot, j_lo = j;
+      uint32_t count = 0;
+      while (j < O && j - j_lo < kMaxFilterOffices && count + cnt[j] <= B) count += cnt[j++];
+      if (count) wb.push_back(AgentBlock{i, first, count, j_lo, j - 1, j_lo, j - 1, 0});
+      slot += count;
+    }
+    for (uint32_t first = slot; first < D.n; first += B)
+      rb.push_back(AgentBlock{i, first, std::min(B, D.n - first), 1, 0, 1, 0, 0});
+    const size_t total = wb.size() + rb.size();
+    for (size_t t = 0, w = 0, r = 0; t < total; ++t) {
+      // block t is the r-th other-agent block when r * total <= t * |rb| (an even spread)
+      if (r < rb.size() && (w == wb.size() || r * total <= t * rb.size())) s->office_blocks.push_back(rb[r++]);
+      else s->office_blocks.push_back(wb[w++]);
+    }
+    D.office_n_blocks = static_cast<uint32_t>(s->office_blocks.size()) - D.office_block_begin;
   }
   s->shape.office_agent_blocks = static_cast<uint32_t>(s->office_blocks.size());
+  if (s->agent_blocks.size() > s->cap_blocks || s->office_blocks.size() > s->cap_office_blocks)
+    return fail(EPI_ERR_DEVICE, "agent block tables exceed their allocation");
+  return EPI_OK;
+}
+
+// The office layout's slot order (model.hpp): per simulation, the Normal and Essential
+// workers office by office (office = id % n_offices), each office's in id order, then the
+// other agents in id order. Work statuses are fixed at creation, so the order is computed
+// once, from the synthesized state words, and uploaded as the slot -> id map.
+int build_office_order(epi_sim* s, const std::vector<Population>& pops, bool host_init) {
+  s->office_workers.assign(s->n_sims, {});
+  std::vector<uint32_t> st, order, slot_of, next;
+  std::vector<TileRef> tiles;
+  for (uint32_t i = 0; i < s->n_sims; ++i) {
+    SimDesc& D = s->desc[i];
+    const uint32_t n = D.n, O = D.n_offices;
+    if (host_init) {
+      st = pops[i].st;
+    } else {
+      st.assign(n, 0);
+      if (n) HIP_TRY(hipMemcpy(st.data(), s->v.st + D.agent_off, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    std::vector<uint32_t>& cnt = s->office_workers[i];
+    cnt.assign(O, 0);
+    const auto worker = [&](uint32_t id) {
+      const uint32_t ws = st_ws(st[id]);
+      return ws == WS_NORMAL || ws == WS_ESSENTIAL;
+    };
+    uint32_t workers = 0;
+    for (uint32_t id = 0; id < n; ++id)
+      if (worker(id)) {
+        ++cnt[id % O];
+        ++workers;
+      }
+    next.assign(O, 0);
+    for (uint32_t j = 1; j < O; ++j) next[j] = next[j - 1] + cnt[j - 1];
+    order.assign(n, 0);
+    slot_of.assign(n, 0);
+    uint32_t rest = workers;
+    for (uint32_t id = 0; id < n; ++id) {
+      const uint32_t slot = worker(id) ? next[id % O]++ : rest++;
+      order[slot] = id;
+      slot_of[id] = slot;
+    }
+    D.n_workers = workers;
+    if (n) {
+      HIP_TRY(hipMemcpy(s->d_office_agent + D.agent_off, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(s->d_office_slot + D.agent_off, slot_of.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    const uint32_t K = (n + O - 1u) / O;  // rows k of ids k * O + j
+    for (uint32_t k0 = 0; k0 < K; k0 += 32)
+      for (uint32_t j0 = 0; j0 < O; j0 += 32) tiles.push_back(TileRef{i, k0, j0});
+  }
+  s->n_tiles = static_cast<uint32_t>(tiles.size());
+  HIP_TRY(s->mem.alloc(&s->d_tiles, tiles.size()));
+  if (!tiles.empty())
+    HIP_TRY(hipMemcpy(s->d_tiles, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   return EPI_OK;
 }
 
 // The device table is allocated for 64-thread blocks (the most blocks any shape needs).
 int upload_tables(epi_sim* s) {
   if (s->shape.agent_blocks)
-    HIP_TRY(hipMemcpyAsync(s->d_agent_blocks, s->agent_blocks.data(), s->agent_blocks.size() * sizeof(BlockRef),
+    HIP_TRY(hipMemcpyAsync(s->d_agent_blocks, s->agent_blocks.data(), s->agent_blocks.size() * sizeof(AgentBlock),
                            hipMemcpyHostToDevice, s->stream));
   if (s->shape.office_agent_blocks)
-    HIP_TRY(hipMemcpyAsync(s->d_office_blocks, s->office_blocks.data(), s->office_blocks.size() * sizeof(BlockRef),
+    HIP_TRY(hipMemcpyAsync(s->d_office_blocks, s->office_blocks.data(), s->office_blocks.size() * sizeof(AgentBlock),
                            hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(s->d_desc, s->desc.data(), s->desc.size() * sizeof(SimDesc), hipMemcpyHostToDevice,
                          s->stream));
@@ -635,10 +726,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     D.fd_house_nx = make_fastdiv(static_cast<uint32_t>(D.house_nx));
     D.fd_offices = make_fastdiv(D.n_offices);
     D.fd_office_nx = make_fastdiv(static_cast<uint32_t>(D.office_nx));
-    // agent slots: n_offices x per_office in both layouts (holes past n)
-    D.per_office = (D.n + D.n_offices - 1u) / D.n_offices;
-    D.n_slots = D.n_offices * D.per_office;
-    D.fd_per_office = make_fastdiv(std::max(D.per_office, 1u));
+    D.n_slots = D.n;
     D.dz = Disease::params(c);
     D.has_lockdown = c.has_lockdown;
     D.lockdown_at = c.lockdown_at_number_of_infections;
@@ -695,8 +783,6 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     const uint32_t last = s->desc[i].hours > 0 ? s->desc[i].hours - 1u : 0u;
     if (last % 24u >= 1u && last % 24u <= 6u) s->night = false;
   }
-  rc = build_launch_tables(s.get());
-  if (rc != EPI_OK) return rc;
 
   // device memory
   HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -715,16 +801,21 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   HIP_TRY(M.alloc(&s->d_desc, s->n_sims));
   v.desc = s->d_desc;
   HIP_TRY(M.alloc(&v.state, s->n_sims));
-  // tables sized for the smallest legal block (64 threads; office blocks of at least 33
-  // slots) so block-size changes never reallocate
+  // tables sized for the smallest legal block (64 threads) so block-size changes never
+  // reallocate: the office layout has at most one packed block per office, one split
+  // block per office and per 64 workers, and one block per 64 other agents
   uint64_t max_blocks = 0, max_office_blocks = 0;
   for (uint32_t i = 0; i < s->n_sims; ++i) {
     max_blocks += (s->desc[i].n_slots + 63) / 64;
-    max_office_blocks += (s->desc[i].n_slots + 31) / 32;
+    max_office_blocks += 2ull * s->desc[i].n_offices + 2ull * ((s->desc[i].n_slots + 63) / 64) + 2;
   }
   HIP_TRY(M.alloc(&s->d_agent_blocks, max_blocks));
   HIP_TRY(M.alloc(&s->d_office_blocks, max_office_blocks));
-  HIP_TRY(M.alloc(&v.block_counts, 6 * max_office_blocks));
+  s->cap_blocks = max_blocks;
+  s->cap_office_blocks = max_office_blocks;
+  HIP_TRY(M.alloc(&s->d_office_agent, s->n_agents));
+  HIP_TRY(M.alloc(&s->d_office_slot, s->n_agents));
+  HIP_TRY(M.alloc(&v.block_counts, 6 * std::max(max_blocks, max_office_blocks)));
   v.agent_blocks = s->d_agent_blocks;
   BlockRef* d_hosp_blocks = nullptr;
   HIP_TRY(M.alloc(&d_hosp_blocks, s->hosp_blocks.size()));
@@ -787,18 +878,18 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     o.pos = vo.pos;
     o.st = vo.st;
     o.agent_blocks = s->d_office_blocks;
+    o.office_agent = s->d_office_agent;
+    o.office_slot = s->d_office_slot;
     o.office = 1;
     v.office = 0;
+    v.office_agent = nullptr;
+    v.office_slot = nullptr;
     vo = o;
   }
-  std::vector<TileRef> tiles;
-  for (uint32_t i = 0; i < s->n_sims; ++i)
-    for (uint32_t k0 = 0; k0 < s->desc[i].per_office; k0 += 32)
-      for (uint32_t j0 = 0; j0 < s->desc[i].n_offices; j0 += 32) tiles.push_back(TileRef{i, k0, j0});
-  s->n_tiles = static_cast<uint32_t>(tiles.size());
-  HIP_TRY(M.alloc(&s->d_tiles, tiles.size()));
-  if (!tiles.empty())
-    HIP_TRY(hipMemcpy(s->d_tiles, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  rc = build_office_order(s.get(), pops, host_init);
+  if (rc != EPI_OK) return rc;
+  rc = build_launch_tables(s.get());
+  if (rc != EPI_OK) return rc;
   rc = upload_tables(s.get());
   if (rc != EPI_OK) return rc;
   HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_active), 2 * s->n_sims * sizeof(uint32_t),

@@ -18,12 +18,13 @@
 //              change no state but flags and areas (citizen/mod.rs:244-248): the epilogue
 //              repeats the last counts and the agent pass applies the pending sweep, sets
 //              non-staff areas to home at hour of day 1 and resets the claim array at 2.
-//   k_relayout: [hours of day 8 and 16] transposes st and pos between the home layout
-//              (slot = agent id) and the office layout (an office's agents in consecutive
-//              slots) that hours of day 8..15 run on, with blocks of whole offices;
-//              there k_step resolves the claims of each office's cells among the
-//              block's agents in LDS, and the winners on offices that lie wholly in one
-//              block (all of them when an office fits a block) claim nothing globally.
+//   k_relayout: [hours of day 8 and 16] moves st and pos between the home layout
+//              (slot = agent id) and the office layout that hours of day 8..15 run on:
+//              each office's workers in consecutive slots, packed whole into blocks,
+//              then everyone else in id order. There k_step resolves the claims of each
+//              office's cells among the block's workers in LDS, and the winners on
+//              offices that lie wholly in one block (all of them when an office fits a
+//              block) claim nothing globally.
 //   k_night / k_quiet_hours: in day graphs, hours of day 1..6 (or 3..6) as one launch
 //              when no sweep can be pending inside them (no vaccination there).
 // A sweep decided at hour h (allocation_map.rs:306-337) is applied by the agent pass of
@@ -49,9 +50,9 @@ constexpr int kWave = 64;
 // earlier claims and only adds a load. (In both layouts the lower ids among an office's or
 // a house's agents, the ones a pre-check looks for, sit in earlier slots.)
 constexpr uint32_t kResidentAgents = 256u * 2048u;
-// Office-hour block claim filter: 100 cells per 10 x 10 office, at most kFilterCells words
-// of LDS (a block of up to 1024 slots covers at most 20 offices when per_office >= 52).
-constexpr uint32_t kOfficeCells = 100u, kFilterCells = 2048u, kNoCell = ~0u;
+// Office-hour block claim filter: 100 cells per 10 x 10 office, for the at most
+// kMaxFilterOffices offices of a block.
+constexpr uint32_t kOfficeCells = 100u, kFilterCells = kMaxFilterOffices * kOfficeCells, kNoCell = ~0u;
 // Target word flags (x and y never exceed 32000, so bits 15 and 31 are free):
 //   kTgtWon   the move has already won (office-hour block claim filter);
 //   kTgtHouse a home-layout move into a cell of the agent's own house, arbitrated in the
@@ -63,15 +64,17 @@ constexpr uint32_t kTgtHouse = 1u << 15;
 constexpr uint32_t kTgtCell = ~(kTgtWon | kTgtHouse);
 constexpr uint32_t kTgtNone = 0u;  // a non-mover: no kTgtHouse bit, so it equals no house move
 
-// Agent id of a slot: the identity in the home layout; slot = office * per_office + k
-// holds agent k * n_offices + office in the office layout.
-__device__ __forceinline__ uint32_t agent_of_slot(const DevView& v, const SimDesc& D, uint32_t slot) {
-  if (!v.office) return slot;
-  const uint32_t office = fast_div(slot, D.fd_per_office);
-  return (slot - office * D.per_office) * D.n_offices + office;
+__device__ __forceinline__ uint32_t first_lane(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Agent id of lane `lane` of a block: the slot itself in the home layout, the office
+// layout's slot -> id map in the office layout; ~0u past the block's slots.
+__device__ __forceinline__ uint32_t agent_of_lane(const DevView& v, const SimDesc& D, const AgentBlock& bm,
+                                                  uint32_t lane) {
+  const uint32_t slot = first_lane(bm.first) + lane;
+  if (lane >= first_lane(bm.count)) return ~0u;
+  return v.office ? v.office_agent[D.agent_off + slot] : slot;
 }
 
-__device__ __forceinline__ uint32_t first_lane(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Makes the values pass through registers at this point: the loads producing them are
 // issued before it and waited for once, together. Without it the compiler sinks each load
@@ -706,7 +709,7 @@ __device__ void run_epilogue(const DevView& v, uint32_t sim, const SimDesc& D, S
 // capped at 72 (a few SGPRs spilled to VGPR lanes) it runs as fast as at 66. Keeping
 // resident a whole second 16-wave block per CU depends on it.
 __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
-  const BlockRef bm = v.agent_blocks[blockIdx.x];
+  const AgentBlock bm = v.agent_blocks[blockIdx.x];
   const uint32_t sim = first_lane(bm.sim);
   const SimDesc& D = v.desc[sim];
   SimState* S = v.state + sim;
@@ -715,9 +718,8 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   const uint32_t hd = h % 24u;
   const Sweep sweep = sweep_slot(*S, (h - 1u) & 1u);  // decided at hour h - 1
   const uint32_t slot = first_lane(bm.first) + threadIdx.x;
-  const uint32_t local = agent_of_slot(v, D, slot);  // the agent id
-  const uint32_t span = v.office ? D.office_span : blockDim.x;  // slots of this block
-  const bool valid = slot < D.n_slots && local < D.n && threadIdx.x < span;
+  const uint32_t local = agent_of_lane(v, D, bm, threadIdx.x);  // the agent id
+  const bool valid = local < D.n;
   const uint8_t* occ = v.occ + D.cell_off;
 
   uint32_t cat = 6;
@@ -728,12 +730,11 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   bool house_move = false;    // a home-layout move into the agent's own house
   const uint64_t gi = D.agent_off + slot;
   uint64_t ci = 0;
-  // Office hours: the offices this block's slots belong to (j_lo .. j_lo + n_off - 1), each
-  // with 100 cells in the block's claim filter; computed before any store.
-  const uint32_t first = first_lane(bm.first);
-  const uint32_t j_lo = v.office ? fast_div(first, D.fd_per_office) : 0u;
-  const uint32_t j_hi = v.office ? min(D.n_offices - 1u, fast_div(first + span - 1u, D.fd_per_office)) : 0u;
-  const bool filter = v.office && (j_hi - j_lo + 1u) * kOfficeCells <= kFilterCells;
+  // Office hours: the offices of this block's workers (j_lo .. j_hi, none in a block of
+  // other agents and in the home layout), each with 100 cells in the block's claim filter.
+  const uint32_t j_lo = first_lane(bm.j_lo), j_hi = first_lane(bm.j_hi);
+  const uint32_t full_lo = first_lane(bm.full_lo), full_hi = first_lane(bm.full_hi);
+  const bool filter = v.office && j_lo <= j_hi && (j_hi - j_lo + 1u) <= kMaxFilterOffices;
   if (valid) {
     // st and pos are loaded together, then the work that needs neither (the buildings and
     // the hour's movement draw, which hour of day 0 does not use: hospitalize draws it for
@@ -756,9 +757,11 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     const uint64_t gid = D.agent_off + local;  // tm and wsa stay in the home layout (by id)
     a.tm_arr = v.tm + gid;
     a.claims = v.claim + D.cell_off;
-    // At office hours the block claim filter below resolves the contended cells instead,
-    // and at the other hours moves into the agent's own house need no claim (below): the
-    // pre-check serves the remaining local moves (transport area, hospital).
+    // At office hours the block claim filter below resolves the contended cells instead
+    // (a pre-check in the blocks of the other agents, whose moves all claim globally, was
+    // measured 1.3% slower), and at the other hours moves into the agent's own house need
+    // no claim (below): the pre-check serves the remaining local moves (transport area,
+    // hospital).
     a.precheck = slot >= kResidentAgents && !v.office;
     const uint32_t w0 = w_stored;
     const bool staff = a.ws() == WS_STAFF;
@@ -793,12 +796,12 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     tgt_word = static_cast<uint32_t>(a.tx) | (static_cast<uint32_t>(a.ty) << 16);
     // a worker's claim of a cell of its own office (the only agents that claim office
     // cells at these hours) enters the filter
-    const uint32_t j = fast_mod(local, D.fd_offices);  // == slot / per_office in the office layout
+    const uint32_t j = fast_mod(local, D.fd_offices);  // the agent's office
     if (filter && moves && a.tx >= a.ox && a.tx <= a.ox + 9 && a.ty >= a.oy && a.ty <= a.oy + 9 && j >= j_lo &&
         j <= j_hi) {
       office_cell = (j - j_lo) * kOfficeCells + static_cast<uint32_t>(a.ty - a.oy) * 10u +
                     static_cast<uint32_t>(a.tx - a.ox);
-      whole_office = j * D.per_office >= first && (j + 1u) * D.per_office <= first + span;
+      whole_office = j >= full_lo && j <= full_hi;
     }
     // Home-layout hours (16..23, 0..7): only a house's own agents (ids congruent mod
     // n_houses) choose cells of that house. Local moves stay in the current area, which is
@@ -816,7 +819,7 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   // meet here: an LDS atomicMin per claimed cell leaves only the block's lowest key, and a
   // lane that is not it has lost to a lower id and makes no global claim. At most one
   // global atomic per cell and block remains, instead of one per rival. On an office whose
-  // slots all lie in this block the filter has seen every claimant, so its survivor has
+  // workers all lie in this block the filter has seen every claimant, so its survivor has
   // won: it makes no global claim and flags its target word for the commit.
   if (filter) {  // uniform
     __shared__ uint32_t s_cell[kFilterCells];
@@ -902,7 +905,7 @@ __global__ void k_night(DevView v, uint32_t n_epi) {
     return;
   }
   const uint32_t b = blockIdx.x - n_epi;
-  const BlockRef bm = v.agent_blocks[b];
+  const AgentBlock bm = v.agent_blocks[b];
   const uint32_t sim = first_lane(bm.sim);
   const SimDesc& D = v.desc[sim];
   const SimState* S = v.state + sim;
@@ -948,7 +951,7 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
     epilogue_block(v, blockIdx.x);
     return;
   }
-  const BlockRef bm = v.agent_blocks[blockIdx.x - n_epi];
+  const AgentBlock bm = v.agent_blocks[blockIdx.x - n_epi];
   const uint32_t sim = first_lane(bm.sim);
   const SimDesc& D = v.desc[sim];
   const SimState* S = v.state + sim;
@@ -959,8 +962,8 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
     rest_pass(v, D, *S, h, slot, blockIdx.x - n_epi - D.block_begin);
     return;
   }
-  const uint32_t local = agent_of_slot(v, D, slot);  // the agent id
-  if (slot < D.n_slots && local < D.n && (!v.office || threadIdx.x < D.office_span)) {
+  const uint32_t local = agent_of_lane(v, D, bm, threadIdx.x);  // the agent id
+  if (local < D.n) {
     const uint64_t gi = D.agent_off + slot;
     // st, pos and tgt are loaded together, and at home-layout hours with them the target
     // words of the agent's lower housemates (ids local - k * n_houses, k = 1..3: a house
@@ -1008,7 +1011,7 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
 // Re-applies each simulation's last decided sweep (idempotent) so that agent state read
 // by the host includes it; the next hour's agent pass applies it again to the same words.
 __global__ void k_apply_sweep(DevView v) {
-  const BlockRef bm = v.agent_blocks[blockIdx.x];
+  const AgentBlock bm = v.agent_blocks[blockIdx.x];
   const uint32_t sim = first_lane(bm.sim);
   const SimDesc& D = v.desc[sim];
   const SimState* S = v.state + sim;
@@ -1051,44 +1054,68 @@ __global__ void k_hospital_scan(DevView v, uint32_t cells_per_thread) {
   if ((threadIdx.x & (kWave - 1)) == 0 && best != ~0u) atomicMin(&S->first_vacant, best);
 }
 
-// Layout transpose of one tile: agents k0..k0+31 (k = id / n_offices) of offices
-// j0..j0+31 (j = id % n_offices), for st and pos (tm and wsa, which few agents read, stay
-// in the home layout and are indexed by id in both). A simulation's home slot
-// k * n_offices + j and office slot j * per_office + k hold the same agent, so the layouts
-// are the two row-major orders of a per_office x n_offices matrix. Reads and writes are
-// both coalesced (32 consecutive words per row); the turn goes through LDS, whose 33-word
-// rows keep the column reads free of bank conflicts.
+// Layout move of st and pos (tm and wsa, which few agents read, stay in the home layout and
+// are indexed by id in both), one tile of 32 x 32 agents id = k * n_offices + j per block.
+// Along j the ids are consecutive: the home side and the slot map are read (or written)
+// 128 B at a time, and so are the office slots of the tile's other agents, which keep id
+// order. Along k lie the workers of one office, which hold consecutive office slots: the
+// workers go through LDS rows of 33 words so that their office side is also accessed a
+// column of the tile at a time.
 __global__ void __launch_bounds__(256) k_relayout(DevView home, DevView office, const TileRef* tiles,
                                                   uint32_t to_office) {
-  __shared__ uint32_t tile[2][32][33];  // [array][k - k0][j - j0]
+  __shared__ uint32_t tile[2][32][33];  // [array][k - k0][j - j0]: workers' words
+  __shared__ uint32_t tslot[32][33];    // office slot of (k, j); ~0u: no agent or not a worker
   const TileRef tr = tiles[blockIdx.x];
   const SimDesc& D = home.desc[first_lane(tr.sim)];
-  const uint32_t K = D.per_office, O = D.n_offices;
+  const uint32_t O = D.n_offices, W = D.n_workers, n = D.n;
   const uint64_t base = D.agent_off;
   const uint32_t k0 = first_lane(tr.k0), j0 = first_lane(tr.j0);
   const uint32_t lx = threadIdx.x & 31u, ly = threadIdx.x >> 5;
-  const DevView& src = to_office ? home : office;
-  const DevView& dst = to_office ? office : home;
+  // along j: the slot map, the home words, and the non-workers' office words
   for (uint32_t r = ly; r < 32u; r += 8u) {
-    // source rows: home row k = k0 + r (columns j), office row j = j0 + r (columns k)
-    const uint32_t k = to_office ? k0 + r : k0 + lx, j = to_office ? j0 + lx : j0 + r;
-    const uint32_t tk = k - k0, tj = j - j0;
-    if (k < K && j < O) {
-      const uint64_t i = base + (to_office ? static_cast<uint64_t>(k) * O + j : static_cast<uint64_t>(j) * K + k);
-      tile[0][tk][tj] = src.st[i];
-      tile[1][tk][tj] = src.pos[i];
+    const uint32_t j = j0 + lx;
+    const uint64_t id = static_cast<uint64_t>(k0 + r) * O + j;
+    uint32_t slot = ~0u;
+    if (j < O && id < n) {
+      slot = office.office_slot[base + id];
+      if (to_office) {
+        const uint32_t w = home.st[base + id], p = home.pos[base + id];
+        if (slot < W) {
+          tile[0][r][lx] = w;
+          tile[1][r][lx] = p;
+        } else {
+          office.st[base + slot] = w;
+          office.pos[base + slot] = p;
+        }
+      } else if (slot >= W) {
+        const uint32_t w = office.st[base + slot], p = office.pos[base + slot];
+        home.st[base + id] = w;
+        home.pos[base + id] = p;
+      }
     }
+    tslot[r][lx] = slot < W ? slot : ~0u;
   }
   __syncthreads();
+  // along k: the workers of office j0 + r, in consecutive office slots
   for (uint32_t r = ly; r < 32u; r += 8u) {
-    // destination rows: office row j = j0 + r (columns k), home row k = k0 + r (columns j)
-    const uint32_t k = to_office ? k0 + lx : k0 + r, j = to_office ? j0 + r : j0 + lx;
-    const uint32_t tk = k - k0, tj = j - j0;
-    if (k < K && j < O) {
-      const uint64_t i = base + (to_office ? static_cast<uint64_t>(j) * K + k : static_cast<uint64_t>(k) * O + j);
-      dst.st[i] = tile[0][tk][tj];
-      dst.pos[i] = tile[1][tk][tj];
+    const uint32_t slot = tslot[lx][r];
+    if (slot == ~0u) continue;
+    if (to_office) {
+      office.st[base + slot] = tile[0][lx][r];
+      office.pos[base + slot] = tile[1][lx][r];
+    } else {
+      tile[0][lx][r] = office.st[base + slot];
+      tile[1][lx][r] = office.pos[base + slot];
     }
+  }
+  if (to_office) return;  // uniform
+  __syncthreads();
+  // along j: the workers' home words
+  for (uint32_t r = ly; r < 32u; r += 8u) {
+    if (tslot[r][lx] == ~0u) continue;
+    const uint64_t id = static_cast<uint64_t>(k0 + r) * O + j0 + lx;
+    home.st[base + id] = tile[0][r][lx];
+    home.pos[base + id] = tile[1][r][lx];
   }
 }
 

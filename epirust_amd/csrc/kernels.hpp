@@ -21,7 +21,9 @@ struct DevView {
   uint32_t* claim;  // per cell: lowest claim key of the day's latest claim hour, ~0u = none
   const SimDesc* desc;
   SimState* state;
-  const BlockRef* agent_blocks;  // per agent block: (sim, first slot); the office view has its own
+  const AgentBlock* agent_blocks;  // per agent block; the office view has its own table
+  const uint32_t* office_agent;    // office view: slot -> agent id (per simulation from agent_off)
+  const uint32_t* office_slot;     // office view: agent id -> slot
   const BlockRef* hosp_blocks;   // per hospital-scan block: (sim, first cell)
   Counts7* rows;
   Event* events;
@@ -31,7 +33,8 @@ struct DevView {
   uint32_t office;         // 1: the agent arrays above are in the office layout
 };
 
-// One tile of a simulation's layout transpose: agents k0..k0+31 of offices j0..j0+31.
+// One tile of a simulation's layout move: agents k * n_offices + j for k in k0..k0+31 and
+// j in j0..j0+31.
 struct TileRef {
   uint32_t sim, k0, j0;
 };
@@ -54,7 +57,7 @@ hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStrea
 hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream);
 hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, hipStream_t stream);
 // Copies st and pos between the home and office layouts (to_office: home -> office), one
-// 32 x 32 tile of (agent k, office j) per block.
+// 32 x 32 tile of (k, j) per block.
 hipError_t launch_relayout(const DevView& home, const DevView& office, const TileRef* tiles, uint32_t n_tiles,
                            bool to_office, hipStream_t stream);
 // The epilogues of n_hours consecutive hours of day 3..6 in one launch, with no agent pass

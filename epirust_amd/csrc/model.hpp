@@ -116,19 +116,17 @@ struct SimDesc {
   uint32_t has_lockdown, lockdown_at;
   uint32_t has_hospital, spread_rate_threshold;
   uint32_t vacc_off, n_vacc;  // sorted, last-entry-wins schedule in the shared vacc arrays
-  // Agent slots. Both layouts of the agent arrays give a simulation n_slots =
-  // n_offices * per_office >= n slots from agent_off. Home layout: slot = id. Office
-  // layout (hours of day 8..15): slot = office * per_office + k holds agent
-  // k * n_offices + office (its office is id % n_offices), so the agents of one office sit
-  // together. Slots whose agent would be >= n are holes in both layouts.
-  uint32_t n_slots, per_office;
-  FastDiv fd_per_office;
+  // Agent slots: n_slots = n from agent_off in both layouts of the agent arrays. Home
+  // layout: slot = id. Office layout (hours of day 8..15): first the workers (Normal and
+  // Essential) of office 0, 1, ... (an agent's office is id % n_offices), each office's in
+  // id order, then every other agent (not working, hospital staff) in id order; the
+  // office_agent array maps slot -> id. Workers spend these hours in their office, the
+  // others near their house, so both groups keep neighbours in neighbouring slots.
+  uint32_t n_slots;
+  uint32_t n_workers;         // office layout: slots 0 .. n_workers-1 hold the office workers
   uint32_t n_blocks;          // agent blocks of this simulation (step / commit), over n_slots
   uint32_t block_begin;       // index of its first agent block
-  // Office-layout hours use a table of their own: blocks of office_span slots, a whole
-  // number of offices (so that each office's slots lie in one block) when that fills at
-  // least 7/8 of the block, else the block size.
-  uint32_t office_span, office_n_blocks, office_block_begin;
+  uint32_t office_n_blocks, office_block_begin;  // the same in the office layout's table
 };
 
 // Mutable per-simulation scalars, owned by the device.
@@ -156,9 +154,23 @@ struct SimState {
   uint64_t night_vacc_thr;
 };
 
-// Launch-table entry: (simulation, first local agent) or (simulation, first hospital cell).
+// Launch-table entry: (simulation, first hospital cell).
 struct BlockRef {
   uint32_t sim, first;
+};
+
+// Most offices one office-layout block holds, so that its claim filter (100 cells per
+// 10 x 10 office) fits its LDS array.
+constexpr uint32_t kMaxFilterOffices = 20;
+
+// Agent-block table entry: slots first .. first + count - 1 of simulation sim. In the
+// office layout a block holds either workers of offices j_lo .. j_hi (every worker of
+// offices full_lo .. full_hi among them) or only agents that are not office workers
+// (j_lo > j_hi); in the home layout j_lo > j_hi too.
+struct AgentBlock {
+  uint32_t sim, first, count;
+  uint32_t j_lo, j_hi, full_lo, full_hi;
+  uint32_t pad;
 };
 
 struct Counts7 {

@@ -25,6 +25,16 @@ EPI_HD void mulhilo32(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
   hi = static_cast<uint32_t>(p >> 32);
 }
 
+// a ^ b ^ c: one v_bitop3_b32 (truth table 0x96) on gfx950, where the compiler otherwise
+// emits two v_xor_b32 for each of the round's two three-way xors.
+EPI_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 EPI_HD u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
@@ -33,8 +43,8 @@ EPI_HD u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, u
     uint32_t hi0, lo0, hi1, lo1;
     mulhilo32(0xD2511F53u, c0, hi0, lo0);
     mulhilo32(0xCD9E8D57u, c2, hi1, lo1);
-    const uint32_t n0 = hi1 ^ c1 ^ k0;
-    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    const uint32_t n0 = xor3(hi1, c1, k0);
+    const uint32_t n2 = xor3(hi0, c3, k1);
     c0 = n0;
     c1 = lo1;
     c2 = n2;
