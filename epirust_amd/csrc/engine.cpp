@@ -280,11 +280,22 @@ int build_launch_tables(epi_sim* s) {
     for (uint32_t first = slot; first < D.n; first += B)
       rb.push_back(AgentBlock{i, first, std::min(B, D.n - first), 1, 0, 1, 0, 0});
     const size_t total = wb.size() + rb.size();
+    std::vector<AgentBlock> merged;
+    merged.reserve(total);
     for (size_t t = 0, w = 0, r = 0; t < total; ++t) {
       // block t is the r-th other-agent block when r * total <= t * |rb| (an even spread)
-      if (r < rb.size() && (w == wb.size() || r * total <= t * rb.size())) s->office_blocks.push_back(rb[r++]);
-      else s->office_blocks.push_back(wb[w++]);
+      if (r < rb.size() && (w == wb.size() || r * total <= t * rb.size())) merged.push_back(rb[r++]);
+      else merged.push_back(wb[w++]);
     }
+    // Each XCD runs one contiguous run of the merged list (table entry t runs on XCD
+    // t % 8), so that neighbouring offices, and neighbouring houses, share its L2.
+    const uint32_t nb = static_cast<uint32_t>(total);
+    uint32_t start[kXcds];
+    for (uint32_t x = 0, acc = 0; x < kXcds; ++x) {
+      start[x] = acc;
+      acc += nb / kXcds + (x < nb % kXcds ? 1u : 0u);
+    }
+    for (uint32_t t = 0; t < nb; ++t) s->office_blocks.push_back(merged[start[t % kXcds] + t / kXcds]);
     D.office_n_blocks = static_cast<uint32_t>(s->office_blocks.size()) - D.office_block_begin;
   }
   s->shape.office_agent_blocks = static_cast<uint32_t>(s->office_blocks.size());
