@@ -26,7 +26,7 @@ SPAN = {"k_commit": 1, "k_quiet_hours": 4, "k_night": 6}
 
 
 def short(name):
-    m = re.search(r"\b(k_\w+)\(", name)
+    m = re.search(r"\b(k_\w+)(?:<[^>]*>)?\(", name)
     if m:
         return m.group(1)
     if "copyBuffer" in name:
