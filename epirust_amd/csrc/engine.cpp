@@ -306,48 +306,19 @@ int build_launch_tables(epi_sim* s) {
 
 // The office layout's slot order (model.hpp): per simulation, the Normal and Essential
 // workers office by office (office = id % n_offices), each office's in id order, then the
-// other agents in id order. Work statuses are fixed at creation, so the order is computed
-// once, from the synthesized state words, and uploaded as the slot -> id map.
-int build_office_order(epi_sim* s, const std::vector<Population>& pops, bool host_init) {
+// other agents in id order. Work statuses are fixed at creation, so the slot maps are
+// built once, on the device from the synthesized state words, with the layout move's tiles.
+int build_office_order(epi_sim* s) {
   s->office_workers.assign(s->n_sims, {});
-  std::vector<uint32_t> st, order, slot_of, next;
   std::vector<TileRef> tiles;
   for (uint32_t i = 0; i < s->n_sims; ++i) {
     SimDesc& D = s->desc[i];
     const uint32_t n = D.n, O = D.n_offices;
-    if (host_init) {
-      st = pops[i].st;
-    } else {
-      st.assign(n, 0);
-      if (n) HIP_TRY(hipMemcpy(st.data(), s->v.st + D.agent_off, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    }
-    std::vector<uint32_t>& cnt = s->office_workers[i];
-    cnt.assign(O, 0);
-    const auto worker = [&](uint32_t id) {
-      const uint32_t ws = st_ws(st[id]);
-      return ws == WS_NORMAL || ws == WS_ESSENTIAL;
-    };
-    uint32_t workers = 0;
-    for (uint32_t id = 0; id < n; ++id)
-      if (worker(id)) {
-        ++cnt[id % O];
-        ++workers;
-      }
-    next.assign(O, 0);
-    for (uint32_t j = 1; j < O; ++j) next[j] = next[j - 1] + cnt[j - 1];
-    order.assign(n, 0);
-    slot_of.assign(n, 0);
-    uint32_t rest = workers;
-    for (uint32_t id = 0; id < n; ++id) {
-      const uint32_t slot = worker(id) ? next[id % O]++ : rest++;
-      order[slot] = id;
-      slot_of[id] = slot;
-    }
-    D.n_workers = workers;
-    if (n) {
-      HIP_TRY(hipMemcpy(s->d_office_agent + D.agent_off, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(s->d_office_slot + D.agent_off, slot_of.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-    }
+    std::string err;
+    const hipError_t e = build_office_maps_on_device(s->v.st + D.agent_off, n, O, s->d_office_agent + D.agent_off,
+                                                     s->d_office_slot + D.agent_off, s->office_workers[i],
+                                                     D.n_workers, s->stream, err);
+    if (e != hipSuccess) return fail(EPI_ERR_DEVICE, "office layout (" + err + "): " + hipGetErrorString(e));
     const uint32_t K = (n + O - 1u) / O;  // rows k of ids k * O + j
     for (uint32_t k0 = 0; k0 < K; k0 += 32)
       for (uint32_t j0 = 0; j0 < O; j0 += 32) tiles.push_back(TileRef{i, k0, j0});
@@ -897,7 +868,7 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
     v.office_slot = nullptr;
     vo = o;
   }
-  rc = build_office_order(s.get(), pops, host_init);
+  rc = build_office_order(s.get());
   if (rc != EPI_OK) return rc;
   rc = build_launch_tables(s.get());
   if (rc != EPI_OK) return rc;

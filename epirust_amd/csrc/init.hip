@@ -304,6 +304,69 @@ hipError_t select_rank(const InitArgs& a, uint32_t r, uint32_t* d_hist, uint32_t
   return hipSuccess;
 }
 
+// ---- office layout slot maps (model.hpp): Normal and Essential workers office by office
+// (office j = id % n_offices), each office's in id order, then the other agents in id order.
+__device__ __forceinline__ bool office_worker(uint32_t w) {
+  const uint32_t ws = (w >> B_WS) & M_WS;
+  return ws == WS_NORMAL || ws == WS_ESSENTIAL;
+}
+
+// One thread per office: its workers (ids j, j + O, ... in order, coalesced over offices).
+__global__ __launch_bounds__(kInitBlock) void k_office_count(const uint32_t* st, uint32_t n, uint32_t O,
+                                                             uint32_t* cnt) {
+  const uint32_t j = blockIdx.x * kInitBlock + threadIdx.x;
+  if (j >= O) return;
+  uint32_t c = 0;
+  for (uint64_t id = j; id < n; id += O) c += office_worker(st[id]) ? 1u : 0u;
+  cnt[j] = c;
+}
+
+__global__ __launch_bounds__(kInitBlock) void k_office_place(const uint32_t* st, uint32_t n, uint32_t O,
+                                                             const uint32_t* base, uint32_t* office_agent,
+                                                             uint32_t* office_slot) {
+  const uint32_t j = blockIdx.x * kInitBlock + threadIdx.x;
+  if (j >= O) return;
+  uint32_t slot = base[j];
+  for (uint64_t id = j; id < n; id += O)
+    if (office_worker(st[id])) {
+      office_agent[slot] = static_cast<uint32_t>(id);
+      office_slot[id] = slot;
+      ++slot;
+    }
+}
+
+constexpr uint32_t kRestChunk = 1024;  // ids per block of the other agents' passes
+
+// Other agents per chunk of kRestChunk ids.
+__global__ __launch_bounds__(kRestChunk) void k_rest_count(const uint32_t* st, uint32_t n, uint32_t* cnt) {
+  const uint32_t id = blockIdx.x * kRestChunk + threadIdx.x;
+  const int rest = id < n && !office_worker(st[id]);
+  const int c = __syncthreads_count(rest);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = static_cast<uint32_t>(c);
+}
+
+// Slot of each other agent: n_workers + the chunk's base + its rank in the chunk (wave
+// ballots, then the waves' totals in LDS).
+__global__ __launch_bounds__(kRestChunk) void k_rest_place(const uint32_t* st, uint32_t n, uint32_t n_workers,
+                                                          const uint32_t* base, uint32_t* office_agent,
+                                                          uint32_t* office_slot) {
+  __shared__ uint32_t wave_total[kRestChunk / 64];
+  const uint32_t id = blockIdx.x * kRestChunk + threadIdx.x;
+  const bool rest = id < n && !office_worker(st[id]);
+  const unsigned long long m = __ballot(rest);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t below = static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+  if (lane == 0) wave_total[wave] = static_cast<uint32_t>(__popcll(m));
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < wave; ++w) before += wave_total[w];
+  if (rest) {
+    const uint32_t slot = n_workers + base[blockIdx.x] + before + below;
+    office_agent[slot] = id;
+    office_slot[id] = slot;
+  }
+}
+
 }  // namespace
 
 hipError_t synthesize_on_device(const Config& cfg, uint64_t seed, const Geometry& geo, const InitTargets& t,
@@ -381,6 +444,54 @@ hipError_t synthesize_on_device(const Config& cfg, uint64_t seed, const Geometry
   INIT_TRY("k_init_grid", hipGetLastError());
   // the temporaries are freed on return: finish with them first
   INIT_TRY("init sync", hipStreamSynchronize(stream));
+  return hipSuccess;
+}
+
+hipError_t build_office_maps_on_device(const uint32_t* st, uint32_t n, uint32_t n_offices, uint32_t* office_agent,
+                                      uint32_t* office_slot, std::vector<uint32_t>& workers_per_office,
+                                      uint32_t& n_workers, hipStream_t stream, std::string& err) {
+  const uint32_t O = std::max(n_offices, 1u);
+  workers_per_office.assign(O, 0);
+  n_workers = 0;
+  if (n == 0) return hipSuccess;
+  const uint32_t chunks = (n + kRestChunk - 1) / kRestChunk;
+  DeviceTemp tmp;
+  uint32_t *d_cnt = nullptr, *d_base = nullptr, *d_rcnt = nullptr, *d_rbase = nullptr;
+  INIT_TRY("office maps: alloc", tmp.alloc(&d_cnt, O));
+  INIT_TRY("office maps: alloc", tmp.alloc(&d_base, O));
+  INIT_TRY("office maps: alloc", tmp.alloc(&d_rcnt, chunks));
+  INIT_TRY("office maps: alloc", tmp.alloc(&d_rbase, chunks));
+  const uint32_t ob = (O + kInitBlock - 1) / kInitBlock;
+  hipLaunchKernelGGL(k_office_count, dim3(ob), dim3(kInitBlock), 0, stream, st, n, O, d_cnt);
+  INIT_TRY("k_office_count", hipGetLastError());
+  hipLaunchKernelGGL(k_rest_count, dim3(chunks), dim3(kRestChunk), 0, stream, st, n, d_rcnt);
+  INIT_TRY("k_rest_count", hipGetLastError());
+  std::vector<uint32_t> rest(chunks), base(O), rbase(chunks);
+  INIT_TRY("office maps: read counts", hipMemcpyAsync(workers_per_office.data(), d_cnt, O * sizeof(uint32_t),
+                                                       hipMemcpyDeviceToHost, stream));
+  INIT_TRY("office maps: read counts",
+           hipMemcpyAsync(rest.data(), d_rcnt, chunks * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+  INIT_TRY("office maps: sync", hipStreamSynchronize(stream));
+  for (uint32_t j = 0; j < O; ++j) {
+    base[j] = n_workers;
+    n_workers += workers_per_office[j];
+  }
+  for (uint32_t c = 0, acc = 0; c < chunks; ++c) {
+    rbase[c] = acc;
+    acc += rest[c];
+  }
+  INIT_TRY("office maps: write bases",
+           hipMemcpyAsync(d_base, base.data(), O * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+  INIT_TRY("office maps: write bases",
+           hipMemcpyAsync(d_rbase, rbase.data(), chunks * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_office_place, dim3(ob), dim3(kInitBlock), 0, stream, st, n, O, d_base, office_agent,
+                     office_slot);
+  INIT_TRY("k_office_place", hipGetLastError());
+  hipLaunchKernelGGL(k_rest_place, dim3(chunks), dim3(kRestChunk), 0, stream, st, n, n_workers, d_rbase,
+                     office_agent, office_slot);
+  INIT_TRY("k_rest_place", hipGetLastError());
+  // the temporaries and the host bases are freed on return: finish with them first
+  INIT_TRY("office maps: sync", hipStreamSynchronize(stream));
   return hipSuccess;
 }
 

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "config.hpp"
 #include "population.hpp"
@@ -28,5 +29,12 @@ struct InitTargets {
 // `err` names the failing step.
 hipError_t synthesize_on_device(const Config& cfg, uint64_t seed, const Geometry& geo, const InitTargets& t,
                                 hipStream_t stream, std::string& err);
+
+// The office layout's slot maps of one simulation (model.hpp) from its home-layout state
+// words `st` (n agents): office_agent[slot] = id and office_slot[id] = slot, with the
+// number of Normal and Essential workers of each office and in all, on `stream`.
+hipError_t build_office_maps_on_device(const uint32_t* st, uint32_t n, uint32_t n_offices, uint32_t* office_agent,
+                                      uint32_t* office_slot, std::vector<uint32_t>& workers_per_office,
+                                      uint32_t& n_workers, hipStream_t stream, std::string& err);
 
 }  // namespace epi
