@@ -374,7 +374,6 @@ struct Agent {
     if (precheck && !in_house(tx, ty)) seen_claim = claims[cell_index(D, tx, ty)];  // overlaps the disease update
   }
 
-  // default_disease_handler.rs:64-86 (OR over fixed per-neighbour draws)
   // default_disease_handler.rs:64-86 as an OR over fixed per-neighbour draws: first the
   // mask of eligible infectious neighbours of (tx, ty) from the window (rows chosen with
   // bit masks, so no variable indexing), then one Philox block per pair of neighbours,
