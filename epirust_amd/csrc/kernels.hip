@@ -129,6 +129,26 @@ __device__ __forceinline__ uint32_t cell_index(const SimDesc& D, int32_t x, int3
 // 3 x 3 neighbourhood masks: bit 3 * (dy + 1) + (dx + 1) stands for the cell at offset
 // (dx, dy), row-major, so bit 4 is the centre. moore8 drops the centre, which leaves the
 // Moore order of point.rs:57-61: (-1,-1) (0,-1) (1,-1) (-1,0) (1,0) (-1,1) (0,1) (1,1).
+// Index of the n-th (0-based) set bit of an 8-bit mask m, n < popcount(m): two halving
+// steps on popcounts and a last compare, the same work for every n (a loop clearing n bits
+// runs as long as the wavefront's largest n).
+__device__ __forceinline__ uint32_t nth_bit8(uint32_t m, uint32_t n) {
+  uint32_t base = 0;
+  const uint32_t c4 = __builtin_popcount(m & 0xFu);
+  if (n >= c4) {
+    n -= c4;
+    m >>= 4;
+    base = 4;
+  }
+  const uint32_t c2 = __builtin_popcount(m & 0x3u);
+  if (n >= c2) {
+    n -= c2;
+    m >>= 2;
+    base += 2;
+  }
+  return base + (n >= (m & 1u) ? 1u : 0u);
+}
+
 __device__ __forceinline__ uint32_t moore8(uint32_t m9) { return (m9 & 0xFu) | ((m9 >> 1) & 0xF0u); }
 
 // Bit j (j = 0..2) set iff lo <= base + j <= hi.
@@ -366,9 +386,7 @@ struct Agent {
     uint32_t cand = moore8(vac9 & allowed9(a, lx, ly));
     const uint32_t n = __builtin_popcount(cand);
     if (n == 0) return;
-    uint32_t pick = below(move_words().w[2], n);
-    for (uint32_t j = 0; j < pick; ++j) cand &= cand - 1;  // index of the pick-th set bit
-    const int k = __builtin_ctz(cand);
+    const int k = static_cast<int>(nth_bit8(cand, below(move_words().w[2], n)));
     tx = lx + moore_dx(k);
     ty = ly + moore_dy(k);
     if (precheck && !in_house(tx, ty)) seen_claim = claims[cell_index(D, tx, ty)];  // overlaps the disease update
