@@ -1286,7 +1286,7 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
       }
       (void)hipEventRecord(e[1], sim->stream);
       had_step[k] = hour_has_step(h) ? 1 : 0;
-      // the layout transposes of hours of day 8 and 16 are timed with the step
+      // the layout moves of hours of day 8 and 16 are timed with the step
       if ((h % 24u == 8 || h % 24u == 16) && rc == EPI_OK) rc = relayout(sim, h % 24u == 8);
       const DevView& v = office_hour(h) ? sim->v_office : sim->v;
       if (had_step[k] && launch_step(v, sim->shape, sim->stream) != hipSuccess) rc = fail(EPI_ERR_DEVICE, "launch");

@@ -87,6 +87,14 @@ def test_block_shapes_do_not_change_results(cfg1, block):
     run_lockstep(cfg, 300, "block%d" % block, every=5, block=block)
 
 
+def test_sparse_offices_fill_blocks_to_the_filter_limit(cfg1):
+    """3,000 agents on 600x600: 720 offices of ~3 workers each, so each office-layout block
+    of workers holds the claim filter's maximum of 20 offices (the other blocks' lanes run
+    the agents that do not work in an office)."""
+    cfg = interventions_config(cfg1, n=3000, g=600)
+    run_lockstep(cfg, 240, "sparse offices", every=8)
+
+
 def test_whole_run_with_day_graphs_matches(cfg1):
     """One epi_step call covering the run: whole days replay the captured hipGraph."""
     say("[parity] cfg1 graph-replayed run")
