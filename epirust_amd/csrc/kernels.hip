@@ -541,6 +541,7 @@ struct Agent {
     load_tm();
     Disease::on_routine_end(D.dz, disease_words(), w, tm);
     if (state() == ST_R) {
+      mv = dr(SITE_MOVE, 0);  // hour of day 23 draws it only here
       int32_t px, py;
       random_point(home(), px, py);
       move_to(px, py);
@@ -754,16 +755,16 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   const bool filter = v.office && j_lo <= j_hi && (j_hi - j_lo + 1u) <= kMaxFilterOffices;
   if (valid) {
     // st and pos are loaded together, then the work that needs neither (the buildings and
-    // the hour's movement draw, which hour of day 0 does not use: hospitalize draws it for
-    // its rare fallback), then one wait (wsa and tm are read by few agents and loaded where
-    // they are needed).
+    // the hour's movement draw, which hours of day 0 and 23 do not use: hospitalize draws it
+    // for its rare fallback, deceased for the recovered, who go home), then one wait (wsa
+    // and tm are read by few agents and loaded where they are needed).
     uint32_t w_stored = v.st[gi];
     uint32_t p = v.pos[gi];
     Agent a(D, *S, occ);
     a.id = local;
     a.h = h;
     a.locate_buildings(hd >= 8 && hd <= 16);
-    if (hd != 0) a.mv = a.dr(SITE_MOVE, 0);
+    if (hd != 0 && hd != 23) a.mv = a.dr(SITE_MOVE, 0);
     loaded_together(w_stored, p, a.mv);
     a.w = sweep.mask ? apply_sweep(w_stored, sweep, local, D) : w_stored;
     a.x = static_cast<int32_t>(p & 0xFFFFu);
