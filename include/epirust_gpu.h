@@ -8,6 +8,11 @@
  * conditions on which the reference engine panics return a nonzero status and
  * epi_last_error() returns the reference's panic text.
  *
+ * The disease model is fixed when the library is built, as the reference's DiseaseHandler
+ * type parameter (engine/src/state_machine/disease_handler.rs:27-44) is fixed when the
+ * engine is compiled: libepigpu.so carries the reference's Disease, and a library built
+ * with another handler type exports these same entry points.
+ *
  * Reference paths below are relative to the reference repository root.
  */
 #ifndef EPIRUST_GPU_H
