@@ -150,6 +150,10 @@ struct epi_sim {
   uint32_t* pinned_active = nullptr;  // 2 x n_sims
   SimState* pinned_state = nullptr;   // n_sims: the states a step call reads back
   Counts7* pinned_rows = nullptr;     // n_rows: staging of the rows a step call returns
+  // the same three buffers' device addresses, which the export kernels write
+  uint32_t* d_pinned_active = nullptr;
+  SimState* d_pinned_state = nullptr;
+  Counts7* d_pinned_rows = nullptr;
   // The home-layout agent arrays hold the current state with the last decided sweep
   // applied; cleared by every step call, restored by settle() before the host reads agents.
   bool settled = true;
@@ -544,10 +548,8 @@ int advance(epi_sim* s, uint32_t target) {
       if (rc != EPI_OK) return rc;
       s->host_hour = end;
     }
-    // copy the active flags of every simulation to pinned memory
-    for (uint32_t i = 0; i < s->n_sims; ++i)
-      HIP_TRY(hipMemcpyAsync(s->pinned_active + slot * s->n_sims + i, &s->v.state[i].active, sizeof(uint32_t),
-                             hipMemcpyDeviceToHost, s->stream));
+    // the active flags of every simulation into pinned memory
+    HIP_TRY(launch_export_active(s->v, s->n_sims, s->d_pinned_active + slot * s->n_sims, s->stream));
     HIP_TRY(hipEventRecord(s->poll_event[slot], s->stream));
     pending[slot] = true;
     ++day_index;
@@ -568,11 +570,6 @@ int record_loop_time(epi_sim* s) {
   return EPI_OK;
 }
 
-// Rows a simulation has room for: one per hour of its loop 1..hours-1.
-uint32_t row_capacity(const epi_sim* s, uint32_t i) {
-  const uint32_t h = s->desc[i].hours;
-  return h > 1 ? h - 1 : 0u;
-}
 
 // The hours are enqueued, then the states and every simulation's rows that this call can
 // have written (at most one per hour enqueued) are copied to pinned memory behind them,
@@ -585,18 +582,9 @@ int step_impl(epi_sim* s, uint32_t n_hours, epi_counts* rows, uint32_t row_strid
   int rc = advance(s, target);
   if (rc != EPI_OK) return rc;
   const uint32_t ran = s->host_hour - start;
-  HIP_TRY(hipMemcpyAsync(s->pinned_state, s->v.state, s->n_sims * sizeof(SimState), hipMemcpyDeviceToHost,
-                         s->stream));
-  if (rows && ran) {
-    for (uint32_t i = 0; i < s->n_sims; ++i) {
-      const uint32_t from = s->rows_returned[i];
-      const uint32_t win = std::min(ran, row_capacity(s, i) - std::min(from, row_capacity(s, i)));
-      if (win == 0) continue;
-      const uint64_t at = s->desc[i].row_off + from;
-      HIP_TRY(hipMemcpyAsync(s->pinned_rows + at, s->v.rows + at, win * sizeof(Counts7), hipMemcpyDeviceToHost,
-                             s->stream));
-    }
-  }
+  // the states, and the rows written since the last export (rows_returned .. rows_written
+  // of each simulation), into pinned memory
+  HIP_TRY(launch_export(s->v, s->n_sims, s->d_pinned_state, s->d_pinned_rows, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   rc = record_loop_time(s);
   if (rc != EPI_OK) return rc;
@@ -880,6 +868,9 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
                         hipHostMallocDefault));
   HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_rows), std::max<uint64_t>(s->n_rows, 1) * sizeof(Counts7),
                         hipHostMallocDefault));
+  HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->d_pinned_active), s->pinned_active, 0));
+  HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->d_pinned_state), s->pinned_state, 0));
+  HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->d_pinned_rows), s->pinned_rows, 0));
   HIP_TRY(hipEventCreateWithFlags(&s->poll_event[0], hipEventDisableTiming));
   HIP_TRY(hipEventCreate(&s->loop_event[0]));
   HIP_TRY(hipEventCreate(&s->loop_event[1]));

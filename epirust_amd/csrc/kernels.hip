@@ -36,6 +36,8 @@
 // integer work bound by HBM traffic.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "disease.hpp"
 #include "kernels.hpp"
 #include "philox.hpp"
@@ -1137,6 +1139,28 @@ __global__ void __launch_bounds__(256) k_relayout(DevView home, DevView office, 
   }
 }
 
+// One block per simulation: its state, then its new rows; then the export mark moves.
+__global__ void __launch_bounds__(256) k_export(DevView v, SimState* states, Counts7* rows) {
+  const uint32_t sim = blockIdx.x;
+  SimState* S = v.state + sim;
+  const SimDesc& D = v.desc[sim];
+  const uint32_t from = S->rows_exported, to = S->rows_written;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(v.rows + D.row_off + from);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(rows + D.row_off + from);
+  const uint32_t words = (to - from) * (sizeof(Counts7) / 4u);
+  for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
+  __syncthreads();  // every lane has read rows_exported before it moves
+  if (threadIdx.x == 0) S->rows_exported = to;
+  const uint32_t* ss = reinterpret_cast<const uint32_t*>(S);
+  uint32_t* sd = reinterpret_cast<uint32_t*>(states + sim);
+  for (uint32_t k = threadIdx.x; k < sizeof(SimState) / 4u; k += blockDim.x) sd[k] = k == offsetof(SimState, rows_exported) / 4u ? to : ss[k];
+}
+
+__global__ void k_export_active(DevView v, uint32_t n_sims, uint32_t* active) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_sims) active[i] = v.state[i].active;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -1187,6 +1211,18 @@ hipError_t launch_relayout(const DevView& home, const DevView& office, const Til
                            bool to_office, hipStream_t stream) {
   if (n_tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_relayout, dim3(n_tiles), dim3(256), 0, stream, home, office, tiles, to_office ? 1u : 0u);
+  return hipGetLastError();
+}
+
+hipError_t launch_export(const DevView& v, uint32_t n_sims, SimState* states, Counts7* rows, hipStream_t stream) {
+  if (n_sims == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_export, dim3(n_sims), dim3(256), 0, stream, v, states, rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_export_active(const DevView& v, uint32_t n_sims, uint32_t* active, hipStream_t stream) {
+  if (n_sims == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_export_active, dim3((n_sims + 255) / 256), dim3(256), 0, stream, v, n_sims, active);
   return hipGetLastError();
 }
 
