@@ -227,6 +227,34 @@ def bench_interventions(G, cpu_seconds):
     return out
 
 
+def bench_hour_at_a_time(G, hours=240):
+    """The integration path that hands every hour's row to the reference's listeners as it
+    is made: epi_step(sim, 1, ...) called once per hour (a stream drain and a read-back per
+    call) over the first `hours` hours of the 1M-agent run, against the same hours in one
+    call."""
+    with open(os.path.join(CONFIGS, "cfg2_1m.json")) as f:
+        cfg = json.load(f)
+    from epirust_amd import autotune
+    block = autotune.tuned_block([cfg])
+    n = cfg["population"]["Auto"]["number_of_agents"]
+    out = {"agents": n, "hours": hours}
+    for name, per_call in (("one_call", hours), ("hour_at_a_time", 1)):
+        s = G.Simulation(cfg)
+        if block:
+            s.set_block_size(block)
+        device_sync()
+        t0 = time.perf_counter()
+        done = 0
+        while done < hours:
+            rows, _ = s.step(per_call)
+            done += len(rows)
+        device_sync()
+        dt = time.perf_counter() - t0
+        s.close()
+        out[name] = {"agent_hours_per_s": n * done / dt, "us_per_hour": 1e6 * dt / done}
+    return out
+
+
 def bench_init(G):
     """epi_create time of the 10M-agent config (allocation + population synthesis + grid),
     with the population synthesized by the device kernels and by the host restatement
@@ -401,6 +429,7 @@ def main():
         result["replicates"] = bench_replicates(G, cpu_s / 3)
         result["interventions_10m"] = bench_interventions(G, cpu_s / 3)
         result["create_10m"] = bench_init(G)
+        result["hour_at_a_time"] = bench_hour_at_a_time(G)
         if not args.skip_surrogate:
             result["surrogate"] = bench_surrogate(args.surrogate_train, args.surrogate_eval, 32, 60)
     print(json.dumps(result))

@@ -548,10 +548,13 @@ int advance(epi_sim* s, uint32_t target) {
       if (rc != EPI_OK) return rc;
       s->host_hour = end;
     }
-    // the active flags of every simulation into pinned memory
-    HIP_TRY(launch_export_active(s->v, s->n_sims, s->d_pinned_active + slot * s->n_sims, s->stream));
-    HIP_TRY(hipEventRecord(s->poll_event[slot], s->stream));
-    pending[slot] = true;
+    // the active flags of every simulation into pinned memory, polled two chunks later
+    // (not after the call's last chunk: the call's read-back has the states)
+    if (s->host_hour < target) {
+      HIP_TRY(launch_export_active(s->v, s->n_sims, s->d_pinned_active + slot * s->n_sims, s->stream));
+      HIP_TRY(hipEventRecord(s->poll_event[slot], s->stream));
+      pending[slot] = true;
+    }
     ++day_index;
   }
   HIP_TRY(hipEventRecord(s->loop_event[1], s->stream));
