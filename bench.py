@@ -42,9 +42,13 @@ def algorithmic_bytes_per_hour(n, g):
     return 32.0 * n + 8.0 * g * g
 
 
-def tuned_block_of(cfg):
+def tuned_shape_of(cfg, block=None):
+    """(home, office) threads per block a handle of this config runs with."""
     from epirust_amd import autotune
-    return autotune.tuned_block([cfg])
+    if block:
+        return block, block
+    default = 512 if cfg["population"]["Auto"]["number_of_agents"] >= (1 << 19) else 256
+    return autotune.tuned_shape([cfg]) or (default, default)
 
 
 def measured_traffic(name):
@@ -132,12 +136,10 @@ def bench_headline(args, G):
     run_hours = cfg["hours"] - 1
 
     from epirust_amd import autotune
-    block = args.block or autotune.tuned_block([cfg])
 
     def make():
         s = G.Simulation(cfg)
-        if block:
-            s.set_block_size(block)
+        autotune.apply_shape(s, [cfg], block=args.block)
         return s
 
     if args.warmup > 0:
@@ -174,17 +176,15 @@ def bench_replicates(G, cpu_seconds):
     from epirust_amd import autotune
     with open(os.path.join(CONFIGS, "cfg4_replicates.json")) as f:
         cfgs = json.load(f)
-    block = autotune.tuned_block(cfgs)
     b = G.Batch(cfgs)
-    if block:
-        b.set_block_size(block)
+    shape = autotune.apply_shape(b, cfgs) or (256, 256)
     dt, rows = timed_steps(b, 2399)
     b.close()
     agent_hours = sum(c["population"]["Auto"]["number_of_agents"] * len(r) for c, r in zip(cfgs, rows))
     out = {"metric": "replicates/s @64 mixed", "value": len(cfgs) / dt, "unit": "replicates/s",
            "seconds": dt, "agent_hours_per_s": agent_hours / dt, "replicates": len(cfgs),
            "total_agents": sum(c["population"]["Auto"]["number_of_agents"] for c in cfgs),
-           "block": block or 256, "hours_per_replicate": 2399}
+           "block": shape[0], "office_block": shape[1], "hours_per_replicate": 2399}
     if cpu_seconds > 0:
         rate, sample = refcpu_rate(cfgs[:4], cpu_seconds)
         full = sum(c["population"]["Auto"]["number_of_agents"] * 2399 for c in cfgs)
@@ -199,9 +199,7 @@ def bench_interventions(G, cpu_seconds):
         cfg = json.load(f)
     from epirust_amd import autotune
     s = G.Simulation(cfg)
-    block = autotune.tuned_block([cfg])
-    if block:
-        s.set_block_size(block)
+    autotune.apply_shape(s, [cfg])
     dt, rows = timed_steps(s, cfg["hours"] - 1)
     device_ms, device_hours = s.last_run_ms()
     ev = s.events()
@@ -235,13 +233,11 @@ def bench_hour_at_a_time(G, hours=240):
     with open(os.path.join(CONFIGS, "cfg2_1m.json")) as f:
         cfg = json.load(f)
     from epirust_amd import autotune
-    block = autotune.tuned_block([cfg])
     n = cfg["population"]["Auto"]["number_of_agents"]
     out = {"agents": n, "hours": hours}
     for name, per_call in (("one_call", hours), ("hour_at_a_time", 1)):
         s = G.Simulation(cfg)
-        if block:
-            s.set_block_size(block)
+        autotune.apply_shape(s, [cfg])
         device_sync()
         t0 = time.perf_counter()
         done = 0
@@ -308,9 +304,9 @@ def profile_driver(args, G):
     import ctypes
     with open(args.config) as f:
         cfg = json.load(f)
+    from epirust_amd import autotune
     s = G.Simulation(cfg)
-    if args.block:
-        s.set_block_size(args.block)
+    autotune.apply_shape(s, [cfg], block=args.block)
     prof = s.profile(args.profile_hours)
     s.close()
     hip = ctypes.CDLL("libamdhip64.so")
@@ -407,7 +403,7 @@ def main():
                                "mode: %d simulated hours timed (hours 1..%d of runs of %d hours)" % (
                                    n, g, g, cfg.get("seed", 0), hours_done, min(hours_done, run_hours), run_hours),
                    "agents": n, "grid_size": g, "hours": run_hours, "hours_timed": hours_done,
-                   "threads_per_block": args.block or tuned_block_of(cfg) or 256,
+                   "threads_per_block": dict(zip(("home", "office"), tuned_shape_of(cfg, args.block))),
                    "kernel_ms": {"hospital_scan": scan_ms, "step": step_ms, "commit": commit_ms}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -87,6 +87,7 @@ SYMBOLS = {
     "epi_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _P(ctypes.c_double), _P(ctypes.c_uint32),
                                    _P(ctypes.c_uint32)]),
     "epi_set_block_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "epi_set_office_block_size": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "epi_last_run_ms": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_double), _P(ctypes.c_uint32)]),
     "epi_citizen_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
                                           _P(ctypes.c_uint32), _P(ctypes.c_uint32)]),
@@ -217,7 +218,12 @@ class _Handle:
                 "hours": hours.value}
 
     def set_block_size(self, threads):
+        """Threads per agent block, home and office layout alike."""
         _check(lib().epi_set_block_size(self._h, threads))
+
+    def set_office_block_size(self, threads):
+        """Threads per agent block of the office layout alone (hours of day 8..15)."""
+        _check(lib().epi_set_office_block_size(self._h, threads))
 
     def last_run_ms(self):
         """(device ms, hours) of the last step call's hour loop, from HIP events on the

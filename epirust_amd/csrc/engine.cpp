@@ -250,15 +250,17 @@ int build_launch_tables(epi_sim* s) {
     }
   }
   s->shape.agent_blocks = static_cast<uint32_t>(s->agent_blocks.size());
-  // Office layout: the workers' blocks hold whole offices, as many consecutive ones as fit
-  // B slots and the claim filter's kMaxFilterOffices; an office of more than B workers
-  // spans blocks of its own. Then blocks of B slots over the other agents. Idle lanes cost
+  // Office layout, in blocks of Bo threads (shape.office_block, tuned apart from B): the
+  // workers' blocks hold whole offices, as many consecutive ones as fit Bo slots and the
+  // claim filter's kMaxFilterOffices; an office of more than Bo workers
+  // spans blocks of its own. Then blocks of Bo slots over the other agents. Idle lanes cost
   // latency hiding, and offices are packed in order, so a block is more than half full
   // unless its next office alone would not fit beside it.
   // The two kinds of blocks are interleaved evenly in dispatch order: the other agents'
   // blocks make more global claims and fewer LDS ones, and run beside the workers' instead
   // of all at the end of the launch.
   s->office_blocks.clear();
+  const uint32_t Bo = s->shape.office_block;
   std::vector<AgentBlock> wb, rb;
   for (uint32_t i = 0; i < s->n_sims && !s->office_workers.empty(); ++i) {
     SimDesc& D = s->desc[i];
@@ -269,20 +271,20 @@ int build_launch_tables(epi_sim* s) {
     uint32_t slot = 0, j = 0;
     const uint32_t O = static_cast<uint32_t>(cnt.size());
     while (j < O) {
-      if (cnt[j] > B) {
-        for (uint32_t k = 0; k < cnt[j]; k += B) wb.push_back(AgentBlock{i, slot + k, std::min(B, cnt[j] - k), j, j, 1, 0, 0});
+      if (cnt[j] > Bo) {
+        for (uint32_t k = 0; k < cnt[j]; k += Bo) wb.push_back(AgentBlock{i, slot + k, std::min(Bo, cnt[j] - k), j, j, 1, 0, 0});
         slot += cnt[j];
         ++j;
         continue;
       }
       const uint32_t first = slot, j_lo = j;
       uint32_t count = 0;
-      while (j < O && j - j_lo < kMaxFilterOffices && count + cnt[j] <= B) count += cnt[j++];
+      while (j < O && j - j_lo < kMaxFilterOffices && count + cnt[j] <= Bo) count += cnt[j++];
       if (count) wb.push_back(AgentBlock{i, first, count, j_lo, j - 1, j_lo, j - 1, 0});
       slot += count;
     }
-    for (uint32_t first = slot; first < D.n; first += B)
-      rb.push_back(AgentBlock{i, first, std::min(B, D.n - first), 1, 0, 1, 0, 0});
+    for (uint32_t first = slot; first < D.n; first += Bo)
+      rb.push_back(AgentBlock{i, first, std::min(Bo, D.n - first), 1, 0, 1, 0, 0});
     const size_t total = wb.size() + rb.size();
     std::vector<AgentBlock> merged;
     merged.reserve(total);
@@ -744,8 +746,10 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   s->shape.n_sims = s->n_sims;
   // Default threads per agent block, as measured on MI355X: 512 for one simulation of at
   // least 2^19 agents (1M agents: 31 against 33 µs per hour at 256; 10M: 231 against 250),
-  // 256 for batches and smaller runs. epi_set_block_size overrides it.
+  // 256 for batches and smaller runs, the same at office hours. epi_set_block_size
+  // overrides both, epi_set_office_block_size then the office layout's alone.
   s->shape.block = (s->n_sims == 1 && s->desc[0].n >= (1u << 19)) ? 512u : 256u;
+  s->shape.office_block = s->shape.block;
   s->quiet_hours = true;
   s->night = true;
   for (uint32_t h : s->vacc_hour) {
@@ -1327,11 +1331,13 @@ int epi_last_run_ms(epi_sim* sim, double* device_ms, uint32_t* hours) {
   return EPI_OK;
 }
 
-int epi_set_block_size(epi_sim* sim, uint32_t threads) {
+static int set_launch_blocks(epi_sim* sim, uint32_t home, uint32_t office) {
   if (!sim) return fail(EPI_ERR_ARG, "sim is null");
-  if (threads < 64 || threads > 1024 || threads % 64) return fail(EPI_ERR_ARG, "block size must be 64..1024, x64");
+  if (office < 64 || office > 1024 || office % 64 || home < 64 || home > 1024 || home % 64)
+    return fail(EPI_ERR_ARG, "block size must be 64..1024, x64");
   HIP_TRY(hipStreamSynchronize(sim->stream));
-  sim->shape.block = threads;
+  sim->shape.block = home;
+  sim->shape.office_block = office;
   int rc = build_launch_tables(sim);
   if (rc != EPI_OK) return rc;
   rc = upload_tables(sim);
@@ -1341,6 +1347,13 @@ int epi_set_block_size(epi_sim* sim, uint32_t threads) {
     sim->day_graph = nullptr;
   }
   return EPI_OK;
+}
+
+int epi_set_block_size(epi_sim* sim, uint32_t threads) { return set_launch_blocks(sim, threads, threads); }
+
+int epi_set_office_block_size(epi_sim* sim, uint32_t threads) {
+  if (!sim) return fail(EPI_ERR_ARG, "sim is null");
+  return set_launch_blocks(sim, sim->shape.block, threads);
 }
 
 }  // extern "C"

@@ -1178,14 +1178,15 @@ hipError_t launch_hospital_scan(const DevView& v, const LaunchShape& s, hipStrea
 hipError_t launch_step(const DevView& v, const LaunchShape& s, hipStream_t stream) {
   const uint32_t blocks = v.office ? s.office_agent_blocks : s.agent_blocks;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_step, dim3(blocks), dim3(s.block), 0, stream, v);
+  hipLaunchKernelGGL(k_step, dim3(blocks), dim3(v.office ? s.office_block : s.block), 0, stream, v);
   return hipGetLastError();
 }
 
 hipError_t launch_commit(const DevView& v, const LaunchShape& s, uint32_t hour, hipStream_t stream) {
   const uint32_t blocks = v.office ? s.office_agent_blocks : s.agent_blocks;
   if (blocks + s.n_sims == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_commit, dim3(s.n_sims + blocks), dim3(s.block), 0, stream, v, hour & 1u, s.n_sims);
+  hipLaunchKernelGGL(k_commit, dim3(s.n_sims + blocks), dim3(v.office ? s.office_block : s.block), 0, stream, v,
+                     hour & 1u, s.n_sims);
   return hipGetLastError();
 }
 

@@ -40,7 +40,8 @@ struct TileRef {
 };
 
 struct LaunchShape {
-  uint32_t block = 256;          // threads per agent block
+  uint32_t block = 256;          // threads per agent block (home layout, and the night launch)
+  uint32_t office_block = 256;   // threads per agent block of the office layout (hours of day 8..15)
   uint32_t agent_blocks = 0;     // total agent blocks over the batch (home layout)
   uint32_t office_agent_blocks = 0;  // the same for the office layout's table
   uint32_t hosp_block = 256;     // threads per hospital-scan block

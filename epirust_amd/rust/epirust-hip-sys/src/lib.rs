@@ -100,6 +100,7 @@ extern "C" {
                        hours_run: *mut u32) -> c_int;
     pub fn epi_last_run_ms(sim: *mut EpiSim, device_ms: *mut f64, hours: *mut u32) -> c_int;
     pub fn epi_set_block_size(sim: *mut EpiSim, threads: u32) -> c_int;
+    pub fn epi_set_office_block_size(sim: *mut EpiSim, threads: u32) -> c_int;
     pub fn epi_last_error() -> *const c_char;
     pub fn epi_destroy(sim: *mut EpiSim);
 

@@ -164,10 +164,16 @@ int epi_profile(epi_sim* sim, uint32_t n_hours, double ms_mean[3], uint32_t laun
  * the citizen-state stream was open). Measurement only; no reference counterpart. */
 int epi_last_run_ms(epi_sim* sim, double* device_ms, uint32_t* hours);
 
-/* Launch shape: threads per agent block (64..1024, multiple of 64). Takes effect on the
- * next step; discards the captured day graph. Default: 512 for a single simulation of at
- * least 2^19 agents, else 256. */
+/* Launch shape (no reference counterpart): threads per agent block (64..1024, multiple of
+ * 64), for the home and the office layout alike. Takes effect on the next step; discards
+ * the captured day graph. Default: 512 for a single simulation of at least 2^19 agents,
+ * else 256. */
 int epi_set_block_size(epi_sim* sim, uint32_t threads);
+
+/* Launch shape (no reference counterpart): threads per agent block of the office layout
+ * alone (hours of day 8..15, whose worker blocks pack whole offices into that many slots),
+ * 64..1024, multiple of 64. epi_set_block_size resets it to the home value. */
+int epi_set_office_block_size(epi_sim* sim, uint32_t threads);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* epi_last_error(void);

@@ -15,11 +15,13 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def run_lockstep(cfg, hours, label, every=1, block=None, seed=None):
+def run_lockstep(cfg, hours, label, every=1, block=None, seed=None, office_block=None):
     say("[parity] %s: lockstep %d hours" % (label, hours))
     gpu = G.Simulation(cfg, seed=seed)
     if block:
         gpu.set_block_size(block)
+    if office_block:
+        gpu.set_office_block_size(office_block)
     cpu = O.OracleSim(cfg, seed=seed)
     assert_agents_equal(gpu.agents(), cpu.agents(), "%s hour 0" % label)
     h = 0
@@ -85,6 +87,15 @@ def test_zero_transmission_closed_forms_through_gpu(cfg1):
 def test_block_shapes_do_not_change_results(cfg1, block):
     cfg = interventions_config(cfg1, n=3000, g=140)
     run_lockstep(cfg, 300, "block%d" % block, every=5, block=block)
+
+
+@pytest.mark.parametrize("home,office", [(512, 64), (128, 1024)])
+def test_office_block_size_apart_from_home_does_not_change_results(cfg1, home, office):
+    """The office layout's blocks (hours of day 8..15) launched at another size than the
+    home layout's. This config has 28 offices of ~75 workers: 64 slots split each of them
+    over two blocks, 1,024 slots pack 13 whole offices into one."""
+    cfg = interventions_config(cfg1, n=3000, g=140)
+    run_lockstep(cfg, 300, "home%d office%d" % (home, office), every=5, block=home, office_block=office)
 
 
 def test_sparse_offices_fill_blocks_to_the_filter_limit(cfg1):
