@@ -325,14 +325,27 @@ int build_office_order(epi_sim* s) {
                                                      s->d_office_slot + D.agent_off, s->office_workers[i],
                                                      D.n_workers, s->stream, err);
     if (e != hipSuccess) return fail(EPI_ERR_DEVICE, "office layout (" + err + "): " + hipGetErrorString(e));
+    // Tiles column by column (32 offices, every k), so that consecutive tiles share the
+    // cache lines of their office slots: an office's workers, split over a column's k
+    // tiles, and the other agents of neighbouring columns, whose slots follow id order.
     const uint32_t K = (n + O - 1u) / O;  // rows k of ids k * O + j
-    for (uint32_t k0 = 0; k0 < K; k0 += 32)
-      for (uint32_t j0 = 0; j0 < O; j0 += 32) tiles.push_back(TileRef{i, k0, j0});
+    for (uint32_t j0 = 0; j0 < O; j0 += 32)
+      for (uint32_t k0 = 0; k0 < K; k0 += 32) tiles.push_back(TileRef{i, k0, j0});
   }
-  s->n_tiles = static_cast<uint32_t>(tiles.size());
-  HIP_TRY(s->mem.alloc(&s->d_tiles, tiles.size()));
-  if (!tiles.empty())
-    HIP_TRY(hipMemcpy(s->d_tiles, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  // Each XCD runs one contiguous run of that order (launch entry t runs on XCD t % 8), so
+  // that the partial lines of neighbouring tiles meet in one L2 before they are written back.
+  const uint32_t nt = static_cast<uint32_t>(tiles.size());
+  uint32_t start[kXcds];
+  for (uint32_t x = 0, acc = 0; x < kXcds; ++x) {
+    start[x] = acc;
+    acc += nt / kXcds + (x < nt % kXcds ? 1u : 0u);
+  }
+  std::vector<TileRef> spread(nt);
+  for (uint32_t t = 0; t < nt; ++t) spread[t] = tiles[start[t % kXcds] + t / kXcds];
+  s->n_tiles = nt;
+  HIP_TRY(s->mem.alloc(&s->d_tiles, spread.size()));
+  if (!spread.empty())
+    HIP_TRY(hipMemcpy(s->d_tiles, spread.data(), spread.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   return EPI_OK;
 }
 

@@ -1080,9 +1080,11 @@ __global__ void k_hospital_scan(DevView v, uint32_t cells_per_thread) {
 // 128 B at a time, and so are the office slots of the tile's other agents, which keep id
 // order. Along k lie the workers of one office, which hold consecutive office slots: the
 // workers go through LDS rows of 33 words so that their office side is also accessed a
-// column of the tile at a time.
+// column of the tile at a time. Each thread owns four rows of the tile and issues the loads
+// of all four before any store, so that a wave keeps four times the bytes in flight.
 __global__ void __launch_bounds__(256) k_relayout(DevView home, DevView office, const TileRef* tiles,
                                                   uint32_t to_office) {
+  constexpr uint32_t kRows = 4;           // rows r = ly + 8 * i of the tile per thread
   __shared__ uint32_t tile[2][32][33];  // [array][k - k0][j - j0]: workers' words
   __shared__ uint32_t tslot[32][33];    // office slot of (k, j); ~0u: no agent or not a worker
   const TileRef tr = tiles[blockIdx.x];
@@ -1091,51 +1093,81 @@ __global__ void __launch_bounds__(256) k_relayout(DevView home, DevView office, 
   const uint64_t base = D.agent_off;
   const uint32_t k0 = first_lane(tr.k0), j0 = first_lane(tr.j0);
   const uint32_t lx = threadIdx.x & 31u, ly = threadIdx.x >> 5;
+  const uint32_t j = j0 + lx;
   // along j: the slot map, the home words, and the non-workers' office words
-  for (uint32_t r = ly; r < 32u; r += 8u) {
-    const uint32_t j = j0 + lx;
-    const uint64_t id = static_cast<uint64_t>(k0 + r) * O + j;
-    uint32_t slot = ~0u;
-    if (j < O && id < n) {
-      slot = office.office_slot[base + id];
+  uint64_t id[kRows];
+  uint32_t slot[kRows], w[kRows], p[kRows];
+#pragma unroll
+  for (uint32_t i = 0; i < kRows; ++i) {
+    id[i] = static_cast<uint64_t>(k0 + ly + 8u * i) * O + j;
+    const bool valid = j < O && id[i] < n;
+    slot[i] = valid ? office.office_slot[base + id[i]] : ~0u;  // ~0u >= W: never a worker
+    if (to_office && valid) {
+      w[i] = home.st[base + id[i]];
+      p[i] = home.pos[base + id[i]];
+    }
+  }
+  if (!to_office) {  // uniform: the non-workers' office words, by slot
+#pragma unroll
+    for (uint32_t i = 0; i < kRows; ++i)
+      if (slot[i] >= W && slot[i] != ~0u) {
+        w[i] = office.st[base + slot[i]];
+        p[i] = office.pos[base + slot[i]];
+      }
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < kRows; ++i) {
+    const uint32_t r = ly + 8u * i;
+    if (slot[i] < W) {
       if (to_office) {
-        const uint32_t w = home.st[base + id], p = home.pos[base + id];
-        if (slot < W) {
-          tile[0][r][lx] = w;
-          tile[1][r][lx] = p;
-        } else {
-          office.st[base + slot] = w;
-          office.pos[base + slot] = p;
-        }
-      } else if (slot >= W) {
-        const uint32_t w = office.st[base + slot], p = office.pos[base + slot];
-        home.st[base + id] = w;
-        home.pos[base + id] = p;
+        tile[0][r][lx] = w[i];
+        tile[1][r][lx] = p[i];
+      }
+    } else if (slot[i] != ~0u) {
+      if (to_office) {
+        office.st[base + slot[i]] = w[i];
+        office.pos[base + slot[i]] = p[i];
+      } else {
+        home.st[base + id[i]] = w[i];
+        home.pos[base + id[i]] = p[i];
       }
     }
-    tslot[r][lx] = slot < W ? slot : ~0u;
+    tslot[r][lx] = slot[i] < W ? slot[i] : ~0u;
   }
   __syncthreads();
   // along k: the workers of office j0 + r, in consecutive office slots
-  for (uint32_t r = ly; r < 32u; r += 8u) {
-    const uint32_t slot = tslot[lx][r];
-    if (slot == ~0u) continue;
-    if (to_office) {
-      office.st[base + slot] = tile[0][lx][r];
-      office.pos[base + slot] = tile[1][lx][r];
-    } else {
-      tile[0][lx][r] = office.st[base + slot];
-      tile[1][lx][r] = office.pos[base + slot];
-    }
+  uint32_t ks[kRows];
+#pragma unroll
+  for (uint32_t i = 0; i < kRows; ++i) ks[i] = tslot[lx][ly + 8u * i];
+  if (to_office) {  // uniform
+#pragma unroll
+    for (uint32_t i = 0; i < kRows; ++i)
+      if (ks[i] != ~0u) {
+        office.st[base + ks[i]] = tile[0][lx][ly + 8u * i];
+        office.pos[base + ks[i]] = tile[1][lx][ly + 8u * i];
+      }
+    return;
   }
-  if (to_office) return;  // uniform
+#pragma unroll
+  for (uint32_t i = 0; i < kRows; ++i)
+    if (ks[i] != ~0u) {
+      w[i] = office.st[base + ks[i]];
+      p[i] = office.pos[base + ks[i]];
+    }
+#pragma unroll
+  for (uint32_t i = 0; i < kRows; ++i)
+    if (ks[i] != ~0u) {
+      tile[0][lx][ly + 8u * i] = w[i];
+      tile[1][lx][ly + 8u * i] = p[i];
+    }
   __syncthreads();
   // along j: the workers' home words
-  for (uint32_t r = ly; r < 32u; r += 8u) {
+#pragma unroll
+  for (uint32_t i = 0; i < kRows; ++i) {
+    const uint32_t r = ly + 8u * i;
     if (tslot[r][lx] == ~0u) continue;
-    const uint64_t id = static_cast<uint64_t>(k0 + r) * O + j0 + lx;
-    home.st[base + id] = tile[0][r][lx];
-    home.pos[base + id] = tile[1][r][lx];
+    home.st[base + id[i]] = tile[0][r][lx];
+    home.pos[base + id[i]] = tile[1][r][lx];
   }
 }
 
