@@ -107,6 +107,24 @@ def peak_and_final(curves):
     return peak, final
 
 
+def mean_within_projection(peaks, finals, bound, seeds_list):
+    """For each ensemble size N in `seeds_list`: the projected fraction of points at which
+    the true mean is within `bound` (relative) of an N-seed ensemble mean on both peak
+    infected and final size. `peaks` and `finals` are (points, seeds) single-run values.
+    Per point, an N-seed mean's relative standard error is the single-run coefficient of
+    variation / sqrt(N) (normal approximation); the two statistics are taken as
+    independent. This is the score of a predictor that knew the true mean curve, and so an
+    upper bound on the share of points a surrogate can answer without simulating them."""
+    erf = np.frompyfunc(math.erf, 1, 1)
+
+    def p_within(x, n):
+        cv = x.std(axis=1, ddof=1) / np.maximum(x.mean(axis=1), 1e-12)
+        z = bound * math.sqrt(n) / np.maximum(cv, 1e-12)
+        return erf(z / math.sqrt(2.0)).astype(np.float64)
+
+    return {str(n): float(np.mean(p_within(peaks, n) * p_within(finals, n))) for n in seeds_list}
+
+
 def simulate(params, seeds, spec, batch=128, block=None, progress=None):
     """Daily curves (len(params), days, 6) from the batched GPU simulator."""
     days = spec["days"]
@@ -395,6 +413,16 @@ def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=100
         hf = np.abs(sf[:, :h].mean(axis=1) - sf[:, h:2 * h].mean(axis=1)) / 2 / np.maximum(sf.mean(axis=1), 1e-9)
         metrics["exact_mean_fraction_within_estimate"] = float(
             ((hp <= spec["max_relative_error"]) & (hf <= spec["max_relative_error"])).mean())
+    # The same bound projected to larger judging ensembles from each point's single-run
+    # spread: how many seeds per point would let a perfect predictor answer 95% / 99% of
+    # the points within the error bound (a sweep that simulates the rest is at most
+    # 1 / (simulated fraction) times faster than the plain sweep).
+    if eval_seeds >= 2:
+        proj = mean_within_projection(sp, sf, spec["max_relative_error"], [32, 64, 128, 256, 512, 1024, 2048, 4096])
+        metrics["exact_mean_fraction_within_projected"] = proj
+        for target in (0.95, 0.99):
+            ns = [int(n) for n, v in proj.items() if v >= target]
+            metrics["seeds_for_exact_mean_within_%d" % round(100 * target)] = min(ns) if ns else None
 
     curves_fb, fb, sweep_seconds = sur.sweep(ex, eval_seeds, block=block)
     efp, eff = relative_errors(curves_fb, ref)
@@ -425,7 +453,8 @@ def run_pipeline(out_dir, n_train=None, n_eval=None, eval_seeds=None, epochs=100
         "test_points_per_s": len(test) / sweep_t, "speedup_vs_plain_sweep": (len(test) / sweep_t) / plain_rate,
     }
     np.savez_compressed(os.path.join(out_dir, "surrogate_eval_points.npz"), unit_x=ex, predicted_err=err_hat,
-                        rel_err_peak=ep_, rel_err_final=ef_, nn_rel_err_peak=np_, nn_rel_err_final=nf_)
+                        rel_err_peak=ep_, rel_err_final=ef_, nn_rel_err_peak=np_, nn_rel_err_final=nf_,
+                        ensemble_peak=sp, ensemble_final=sf)
     flat = [m.params() for m in sur.members]
     np.savez_compressed(os.path.join(out_dir, "surrogate_weights.npz"), member0=flat[0], member1=flat[1],
                         y_mean=sur.y_mean, y_std=sur.y_std, noise_x=sur.noise.x, noise_resid=sur.noise.r)

@@ -1,5 +1,6 @@
 """Host logic of the surrogate pipeline (epirust_amd/surrogate.py), no GPU: daily sampling,
-the peak / final-size definitions, the parameter-point config and the calibrated gate."""
+the peak / final-size definitions, the parameter-point config, the calibrated gate and the
+projection of the judging ensemble's noise to other ensemble sizes."""
 import numpy as np
 import pytest
 
@@ -127,3 +128,40 @@ def test_predicted_error_adds_disagreement_and_ensemble_standard_error():
     # without a noise model the error is the disagreement alone
     _, err3 = S.Surrogate(spec, [_Member(d), _Member(-d)], y, np.ones(12)).predict(x)
     assert err3[0] == pytest.approx(0.06 / 0.2)
+
+
+def test_mean_within_projection_closed_form():
+    """One point whose single-run values alternate m - d, m + d (sample CV set exactly) and
+    a final size with no spread: the projected fraction is P(|Z| < bound * sqrt(N) / cv)
+    of a standard normal, here 0.95 at the N where that ratio is 1.959964."""
+    from scipy.stats import norm
+    bound, k, m = 0.05, 32, 0.2
+    target_cv = bound * np.sqrt(128) / norm.ppf(0.975)  # makes N = 128 the 95% point
+    d = target_cv * m / np.sqrt(k / (k - 1))             # std(ddof=1) of +-d is d * sqrt(k / (k - 1))
+    peaks = np.array([[m - d, m + d] * (k // 2)])
+    finals = np.full((1, k), 0.5)
+    proj = S.mean_within_projection(peaks, finals, bound, [32, 128, 512])
+    assert proj["128"] == pytest.approx(0.95, abs=1e-9)
+    for n in (32, 512):
+        assert proj[str(n)] == pytest.approx(2 * norm.cdf(bound * np.sqrt(n) / target_cv) - 1, abs=1e-9)
+    assert proj["32"] < proj["128"] < proj["512"]
+
+
+def test_mean_within_projection_matches_simulated_ensembles():
+    """Points with log-spread coefficients of variation: the projection at N agrees with
+    the fraction of simulated N-seed means (of normal single runs around a known mean)
+    that land within the bound of that mean on both statistics."""
+    rng = np.random.default_rng(7)
+    pts, k, bound = 400, 64, 0.05
+    mean_p, mean_f = rng.uniform(0.05, 0.3, pts), rng.uniform(0.2, 0.9, pts)
+    cv_p, cv_f = np.exp(rng.uniform(np.log(0.02), np.log(0.6), pts)), np.exp(rng.uniform(np.log(0.01), np.log(0.3), pts))
+    # the projection from k sampled runs per point
+    peaks = mean_p[:, None] * (1 + cv_p[:, None] * rng.standard_normal((pts, k)))
+    finals = mean_f[:, None] * (1 + cv_f[:, None] * rng.standard_normal((pts, k)))
+    proj = S.mean_within_projection(peaks, finals, bound, [32, 256])
+    for n in (32, 256):
+        trials = 200
+        ep = np.abs(cv_p[:, None] * rng.standard_normal((pts, trials)) / np.sqrt(n))
+        ef = np.abs(cv_f[:, None] * rng.standard_normal((pts, trials)) / np.sqrt(n))
+        sim = ((ep <= bound) & (ef <= bound)).mean()
+        assert proj[str(n)] == pytest.approx(sim, abs=0.03), (n, proj[str(n)], sim)
