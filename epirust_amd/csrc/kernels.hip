@@ -739,7 +739,10 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
   const Sweep sweep = sweep_slot(*S, (h - 1u) & 1u);  // decided at hour h - 1
   const uint32_t slot = first_lane(bm.first) + threadIdx.x;
   const uint32_t local = agent_of_lane(v, D, bm, threadIdx.x);  // the agent id
-  const bool valid = local < D.n;
+  // Every slot below the block's count holds an agent (the office slot map is a
+  // permutation of the ids), so validity does not wait for the slot -> id read, and the
+  // agent words below are loaded alongside it.
+  const bool valid = threadIdx.x < first_lane(bm.count);
   const uint8_t* occ = v.occ + D.cell_off;
 
   uint32_t cat = 6;
@@ -983,7 +986,7 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
     return;
   }
   const uint32_t local = agent_of_lane(v, D, bm, threadIdx.x);  // the agent id
-  if (local < D.n) {
+  if (threadIdx.x < first_lane(bm.count)) {  // a valid slot, known without the slot -> id read
     const uint64_t gi = D.agent_off + slot;
     // st, pos and tgt are loaded together, and at home-layout hours with them the target
     // words of the agent's lower housemates (ids local - k * n_houses, k = 1..3: a house
