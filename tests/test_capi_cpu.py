@@ -135,3 +135,11 @@ def test_csv_population_records_are_validated(cfg1, tmp_path):
     p.write_text("ind,age,working,pub_transport\n0,30,True,False\n")
     rc, msg, _ = _create(json.dumps(c))
     assert rc in (0, 3), msg  # parses; only the device check is left without a GPU
+
+
+def test_launch_shape_setters_reject_a_null_handle():
+    """Both launch-shape setters check their handle before touching the device."""
+    L = E.lib()
+    for name in ("epi_set_block_size", "epi_set_office_block_size"):
+        assert getattr(L, name)(None, 256) == 4, name  # EPI_ERR_ARG
+        assert L.epi_last_error().decode() == "sim is null"
