@@ -3,6 +3,7 @@
     python -m epirust_amd.prof_summary trace <kernel_stats.csv> [<kernel_trace.csv> --hours H] [--command "..."]
     python -m epirust_amd.prof_summary traffic <fetch.csv> <write.csv> --agents N --grid G --hours H [--command "..."]
     python -m epirust_amd.prof_summary counters <counter_collection.csv>... [--command "..."]
+    python -m epirust_amd.prof_summary hourly <counter_collection.csv>... [--command "..."]
     python -m epirust_amd.prof_summary gemm <kernel_stats.csv> --shape M,N,K [--command "..."]
 
 `trace` turns a `--kernel-trace --stats` kernel_stats.csv into a markdown table (calls,
@@ -185,6 +186,45 @@ def counters(paths, command, clock_ghz=2.4):
     return "\n".join(out) + "\n"
 
 
+def hourly(paths, command):
+    """Per hour of day, the mean per dispatch of each counter of k_step and k_commit, from
+    counter_collection.csv files of `bench.py --profile-hours H` (every hour dispatched on
+    its own; hours are counted by k_commit dispatches), with the L2 hit rate where
+    TCC_HIT / TCC_MISS are present."""
+    per = collections.defaultdict(lambda: collections.defaultdict(lambda: collections.defaultdict(list)))
+    names = set()
+    for path in paths:
+        disp = collections.OrderedDict()
+        for r in csv.DictReader(open(path)):
+            d = int(r["Dispatch_Id"])
+            disp.setdefault(d, [short(r["Kernel_Name"]), collections.defaultdict(float)])
+            disp[d][1][r["Counter_Name"]] += float(r["Counter_Value"])
+        hour = 1
+        for d in sorted(disp):
+            k, vals = disp[d]
+            if k in ("k_step", "k_commit"):
+                for c, v in vals.items():
+                    per[hour % 24][k][c].append(v)
+                    names.add(c)
+            if k == "k_commit":
+                hour += 1
+    names = sorted(names)
+    hit = "TCC_HIT_sum" in names and "TCC_MISS_sum" in names
+    out = ["# Counters per hour of day", "", "Commands:", ""] + ["    " + c for c in command.split(" && ")] + [
+        "", "| hour of day | kernel | " + " | ".join(names) + (" | L2 hit |" if hit else " |"),
+        "|---" * (len(names) + 2 + (1 if hit else 0)) + "|"]
+    for hd in list(range(7, 24)) + [0]:
+        for k in ("k_step", "k_commit"):
+            if not per[hd][k]:
+                continue
+            mean = {c: sum(v) / len(v) for c, v in per[hd][k].items()}
+            row = "| %d | %s | " % (hd, k) + " | ".join("%.4g" % mean.get(c, 0) for c in names)
+            if hit:
+                row += " | %.0f%%" % (100 * mean["TCC_HIT_sum"] / max(mean["TCC_HIT_sum"] + mean["TCC_MISS_sum"], 1))
+            out.append(row + " |")
+    return "\n".join(out) + "\n"
+
+
 def gemm(stats_csv, m, n, k, command, peak_tflops=2500.0):
     """k_gemm's mean duration from a kernel_stats.csv, as TFLOP/s and a fraction of the
     bf16 dense MFMA peak."""
@@ -201,7 +241,7 @@ def gemm(stats_csv, m, n, k, command, peak_tflops=2500.0):
 
 def main(argv):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["trace", "traffic", "counters", "gemm"])
+    ap.add_argument("mode", choices=["trace", "traffic", "counters", "hourly", "gemm"])
     ap.add_argument("--shape", default="4096,512,512", help="gemm: M,N,K")
     ap.add_argument("paths", nargs="+")
     ap.add_argument("--command", default="")
@@ -215,6 +255,8 @@ def main(argv):
         text = trace(a.paths[0], a.command, a.paths[1] if len(a.paths) > 1 else None, a.hours)
     elif a.mode == "counters":
         text = counters(a.paths, a.command)
+    elif a.mode == "hourly":
+        text = hourly(a.paths, a.command)
     elif a.mode == "gemm":
         m, n, k = (int(x) for x in a.shape.split(","))
         text = gemm(a.paths[0], m, n, k, a.command)

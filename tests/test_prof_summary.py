@@ -65,3 +65,37 @@ def test_hours_are_counted_by_commits_and_quiet_dispatches(tmp_path):
             assert all(r[hd][-1] == 2.2 for hd in (3, 4, 5, 6))
         else:
             assert all(r[hd][cols.index("k_commit")] == 4.0 for hd in (3, 4, 5, 6))
+
+
+def test_hourly_counters_follow_the_commit_count(tmp_path):
+    """Two days of per-hour dispatches (--profile-hours: hours 1..6 as commits alone): each
+    k_step / k_commit counter value lands on its hour of day, averaged over the days, and
+    the L2 hit rate is TCC_HIT / (TCC_HIT + TCC_MISS)."""
+    path = tmp_path / "counters.csv"
+    fields = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=fields)
+        w.writeheader()
+        d = 0
+        for day_i in range(2):
+            for h in range(1, 25):
+                hd = h % 24
+                ks = ["k_commit"] if 1 <= hd <= 6 else ["k_step", "k_commit"]
+                for k in ks:
+                    d += 1
+                    hit = 100 * hd + (10 if k == "k_commit" else 0) + day_i  # day 0 and 1 differ by 1
+                    w.writerow({"Dispatch_Id": d, "Kernel_Name": NAME % k, "Counter_Name": "TCC_HIT_sum",
+                                "Counter_Value": hit})
+                    w.writerow({"Dispatch_Id": d, "Kernel_Name": NAME % k, "Counter_Name": "TCC_MISS_sum",
+                                "Counter_Value": 3 * hit})
+    text = P.hourly([str(path)], "cmd")
+    table = {}
+    for line in text.splitlines():
+        cells = [c.strip() for c in line.strip("|").split("|")]
+        if len(cells) == 5 and cells[0].isdigit():
+            table[(int(cells[0]), cells[1])] = (float(cells[2]), float(cells[3]), cells[4])
+    # four significant digits are printed
+    assert table[(9, "k_step")] == (900.5, 2702.0, "25%")
+    assert table[(9, "k_commit")] == (910.5, 2732.0, "25%")
+    assert table[(0, "k_step")] == (0.5, 1.5, "25%")
+    assert (3, "k_commit") not in table  # only hours of day 7..23 and 0 are listed
