@@ -46,12 +46,6 @@ namespace epi {
 namespace {
 
 constexpr int kWave = 64;
-// Agent slots of one simulation that the chip holds at once (256 CUs x 2048 threads).
-// Blocks are dispatched in slot order, so slots below this run in the first residency
-// round, concurrently with every slot before them: a claim pre-check there finds no
-// earlier claims and only adds a load. (In both layouts the lower ids among an office's or
-// a house's agents, the ones a pre-check looks for, sit in earlier slots.)
-constexpr uint32_t kResidentAgents = 256u * 2048u;
 // Office-hour block claim filter: 100 cells per 10 x 10 office, for the at most
 // kMaxFilterOffices offices of a block.
 constexpr uint32_t kOfficeCells = 100u, kFilterCells = kMaxFilterOffices * kOfficeCells, kNoCell = ~0u;
@@ -210,15 +204,6 @@ struct Agent {
   uint32_t id, h, w, tm, wsa;
   uint32_t tm0 = 0;  // tm as stored: loaded for the states that carry an at_hour, else 0
   const uint32_t* tm_arr = nullptr;  // this agent's stored tm word
-  const uint32_t* claims = nullptr;  // this simulation's claim words
-  // Claim word of the chosen cell as read right after a local move is chosen (~0u when not
-  // read). Claim words only decrease within an hour and every word left from earlier hours
-  // (or the daily reset) exceeds any of this hour's keys, so a value below this agent's key,
-  // however stale, means a lower id has claimed the cell: the agent has lost and skips its
-  // claim. Contended cells are mostly claimed first by lower ids in earlier blocks, so for
-  // agents past the first residency round this removes most same-address atomics.
-  uint32_t seen_claim = ~0u;
-  bool precheck = false;  // this agent's block runs after the first residency round
   int32_t x, y;                     // current cell
   int32_t tx, ty;                   // chosen cell
   int32_t hx = 0, hy = 0;           // home house origin (house id % H)
@@ -391,7 +376,6 @@ struct Agent {
     const int k = static_cast<int>(nth_bit8(cand, below(move_words().w[2], n)));
     tx = lx + moore_dx(k);
     ty = ly + moore_dy(k);
-    if (precheck && !in_house(tx, ty)) seen_claim = claims[cell_index(D, tx, ty)];  // overlaps the disease update
   }
 
   // default_disease_handler.rs:64-86 as an OR over fixed per-neighbour draws: first the
@@ -779,13 +763,6 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     a.tm = 0;
     const uint64_t gid = D.agent_off + local;  // tm and wsa stay in the home layout (by id)
     a.tm_arr = v.tm + gid;
-    a.claims = v.claim + D.cell_off;
-    // At office hours the block claim filter below resolves the contended cells instead
-    // (a pre-check in the blocks of the other agents, whose moves all claim globally, was
-    // measured 1.3% slower), and at the other hours moves into the agent's own house need
-    // no claim (below): the pre-check serves the remaining local moves (transport area,
-    // hospital).
-    a.precheck = slot >= kResidentAgents && !v.office;
     const uint32_t w0 = w_stored;
     const bool staff = a.ws() == WS_STAFF;
     a.wsa = staff ? v.wsa[gid] : 0u;
@@ -806,7 +783,7 @@ __global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
     // Everything the stores need is computed first: a descriptor field read after a store
     // can no longer come from the scalar cache and costs a vector round trip.
     key = claim_key(h, local);
-    moves = (a.tx != a.x || a.ty != a.y) && key < a.seen_claim;
+    moves = a.tx != a.x || a.ty != a.y;
     // the grid byte depends only on the state, the hospitalized flag, the infection day and
     // the immunity, so it is recomputed only when one of those changed this hour
     constexpr uint32_t kByteFields = (M_STATE << B_STATE) | F_HOSP | (M_DAY << B_DAY) | (M_IMM << B_IMM);
