@@ -707,12 +707,7 @@ __device__ void run_epilogue(const DevView& v, uint32_t sim, const SimDesc& D, S
   if (threadIdx.x == 0) hour_epilogue(v, in, S, h, s_sum);
 }
 
-// At most 72 SGPRs. Measured at 1024-thread blocks: at 76 SGPRs (the compiler's own
-// choice, once the Philox round keys were kept live across the kernel) every k_step hour
-// ran 10-25% slower than at 66, although the compiler reports 8 waves per SIMD for both;
-// capped at 72 (a few SGPRs spilled to VGPR lanes) it runs as fast as at 66. Keeping
-// resident a whole second 16-wave block per CU depends on it.
-__global__ void __attribute__((amdgpu_num_sgpr(72))) k_step(DevView v) {
+__global__ void k_step(DevView v) {
   const AgentBlock bm = v.agent_blocks[blockIdx.x];
   const uint32_t sim = first_lane(bm.sim);
   const SimDesc& D = v.desc[sim];
