@@ -53,12 +53,14 @@ constexpr uint32_t kOfficeCells = 100u, kFilterCells = kMaxFilterOffices * kOffi
 //   kTgtWon   the move has already won (office-hour block claim filter);
 //   kTgtHouse a home-layout move into a cell of the agent's own house, arbitrated in the
 //             commit among the house's agents instead of by a global claim.
-// At home-layout step hours every agent writes its target word, movers or not, so the
-// words its housemates read in the commit are this hour's.
+// Every agent writes its target word at every step hour, movers or not, so the commit
+// tells movers by it (no flag in the state word, which then changes, and is written, only
+// when the agent's state does) and the words housemates read are this hour's.
 constexpr uint32_t kTgtWon = 1u << 31;
 constexpr uint32_t kTgtHouse = 1u << 15;
 constexpr uint32_t kTgtCell = ~(kTgtWon | kTgtHouse);
-constexpr uint32_t kTgtNone = 0u;  // a non-mover: no kTgtHouse bit, so it equals no house move
+// A non-mover: its cell part (x = y = 32767) is no cell, so it equals no move's word.
+constexpr uint32_t kTgtNone = ~0u;
 
 __device__ __forceinline__ uint32_t first_lane(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -827,12 +829,11 @@ __global__ void k_step(DevView v) {
   }
   if (valid) {
     if (moves) {
-      w_out |= F_MOVE;
       if (whole_office) tgt_word |= kTgtWon;
       else if (house_move) tgt_word |= kTgtHouse;
       else atomicMin(&v.claim[ci], key);
     }
-    if (moves || !v.office) v.tgt[gi] = moves ? tgt_word : kTgtNone;
+    v.tgt[gi] = moves ? tgt_word : kTgtNone;
     if (w_out != w_in) v.st[gi] = w_out;
   }
 
@@ -977,12 +978,13 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
       loaded_together(m1, m2, m3);
     }
     loaded_together(w, p, t);
-    if (w & (F_MOVE | F_DIRTY)) {
+    const bool mover = t != kTgtNone;
+    if (mover || (w & F_DIRTY)) {
       uint8_t* occ = v.occ + D.cell_off;
       const uint32_t old_ci = cell_index(D, static_cast<int32_t>(p & 0xFFFFu), static_cast<int32_t>(p >> 16));
       const uint8_t byte = occupant_byte(D.dz, w);
       bool moved = false;
-      if (w & F_MOVE) {
+      if (mover) {
         // won: decided by the office-hour claim filter; a house move wins unless a lower
         // housemate chose the same cell (the same word, kTgtHouse included); any other
         // move reads back its claim
@@ -998,7 +1000,7 @@ __global__ void k_commit(DevView v, uint32_t parity, uint32_t n_epi) {
         }
       }
       if (!moved && (w & F_DIRTY)) occ[old_ci] = byte;
-      v.st[gi] = w & ~(F_MOVE | F_DIRTY);
+      if (w & F_DIRTY) v.st[gi] = w & ~F_DIRTY;
     }
   }
 }

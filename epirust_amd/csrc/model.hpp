@@ -44,7 +44,7 @@ constexpr uint32_t F_WQ = 1u << 8;
 constexpr uint32_t B_WS = 9, M_WS = 0x3u;
 constexpr uint32_t B_AREA = 11, M_AREA = 0x7u;
 constexpr uint32_t F_PT = 1u << 14;
-constexpr uint32_t F_MOVE = 1u << 15;   // transient: agent targets another cell this hour
+// bit 15 is unused
 constexpr uint32_t F_DIRTY = 1u << 16;  // transient: the agent's grid byte changes this hour
 constexpr uint32_t B_IMM = 17, M_IMM = 0x7u;  // immunity + 2
 constexpr uint32_t B_DAY = 20, M_DAY = 0xFFFu;
