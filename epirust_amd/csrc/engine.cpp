@@ -589,9 +589,8 @@ int record_loop_time(epi_sim* s) {
 }
 
 
-// The hours are enqueued, then the states and every simulation's rows that this call can
-// have written (at most one per hour enqueued) are copied to pinned memory behind them,
-// and the stream is drained once.
+// The hours are enqueued and the stream is drained once. Each hour's epilogue has written
+// its row and its simulation's state into pinned memory, where they are read.
 int step_impl(epi_sim* s, uint32_t n_hours, epi_counts* rows, uint32_t row_stride, uint32_t* n_written,
               int* stopped) {
   const uint64_t want = static_cast<uint64_t>(s->host_hour) + n_hours;
@@ -600,9 +599,6 @@ int step_impl(epi_sim* s, uint32_t n_hours, epi_counts* rows, uint32_t row_strid
   int rc = advance(s, target);
   if (rc != EPI_OK) return rc;
   const uint32_t ran = s->host_hour - start;
-  // the states, and the rows written since the last export (rows_returned .. rows_written
-  // of each simulation), into pinned memory
-  HIP_TRY(launch_export(s->v, s->n_sims, s->d_pinned_state, s->d_pinned_rows, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   rc = record_loop_time(s);
   if (rc != EPI_OK) return rc;
@@ -891,6 +887,10 @@ int create_impl(const std::vector<std::string>& texts, const std::vector<uint64_
   HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->d_pinned_active), s->pinned_active, 0));
   HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->d_pinned_state), s->pinned_state, 0));
   HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&s->d_pinned_rows), s->pinned_rows, 0));
+  // every hour's epilogue writes its row and the state after it here (both views)
+  std::memcpy(s->pinned_state, s->state_h.data(), s->n_sims * sizeof(SimState));
+  s->v.host_rows = s->v_office.host_rows = s->d_pinned_rows;
+  s->v.host_state = s->v_office.host_state = s->d_pinned_state;
   HIP_TRY(hipEventCreateWithFlags(&s->poll_event[0], hipEventDisableTiming));
   HIP_TRY(hipEventCreate(&s->loop_event[0]));
   HIP_TRY(hipEventCreate(&s->loop_event[1]));

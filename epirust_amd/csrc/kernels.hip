@@ -580,7 +580,9 @@ __device__ __forceinline__ void load_epilogue_in(const DevView& v, uint32_t sim,
 __device__ void hour_epilogue(const DevView& v, EpilogueIn& in, SimState* S, uint32_t h, const uint32_t* c) {
   const EpiDesc& D = in.D;
   SimState& st = in.st;
-  v.rows[D.row_off + (h - 1)] = Counts7{h, c[0], c[1], c[2], c[3], c[4], c[5]};
+  const Counts7 row{h, c[0], c[1], c[2], c[3], c[4], c[5]};
+  v.rows[D.row_off + (h - 1)] = row;
+  v.host_rows[D.row_off + (h - 1)] = row;  // what a step call returns, without a copy kernel
   const uint32_t infected = c[2];
   uint32_t sweep = 0;
   unsigned long long vacc_thr = 0;
@@ -647,6 +649,7 @@ __device__ void hour_epilogue(const DevView& v, EpilogueIn& in, SimState* S, uin
     st.commit_hour[1] = next;
   }
   *S = st;
+  v.host_state[S - v.state] = st;
 }
 
 // A decided sweep (allocation_map.rs:349-387): vaccination of susceptible agents (a
@@ -1148,23 +1151,6 @@ __global__ void __launch_bounds__(256) k_relayout(DevView home, DevView office, 
   }
 }
 
-// One block per simulation: its state, then its new rows; then the export mark moves.
-__global__ void __launch_bounds__(256) k_export(DevView v, SimState* states, Counts7* rows) {
-  const uint32_t sim = blockIdx.x;
-  SimState* S = v.state + sim;
-  const SimDesc& D = v.desc[sim];
-  const uint32_t from = S->rows_exported, to = S->rows_written;
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(v.rows + D.row_off + from);
-  uint32_t* dst = reinterpret_cast<uint32_t*>(rows + D.row_off + from);
-  const uint32_t words = (to - from) * (sizeof(Counts7) / 4u);
-  for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) dst[k] = src[k];
-  __syncthreads();  // every lane has read rows_exported before it moves
-  if (threadIdx.x == 0) S->rows_exported = to;
-  const uint32_t* ss = reinterpret_cast<const uint32_t*>(S);
-  uint32_t* sd = reinterpret_cast<uint32_t*>(states + sim);
-  for (uint32_t k = threadIdx.x; k < sizeof(SimState) / 4u; k += blockDim.x) sd[k] = k == offsetof(SimState, rows_exported) / 4u ? to : ss[k];
-}
-
 __global__ void k_export_active(DevView v, uint32_t n_sims, uint32_t* active) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_sims) active[i] = v.state[i].active;
@@ -1221,12 +1207,6 @@ hipError_t launch_relayout(const DevView& home, const DevView& office, const Til
                            bool to_office, hipStream_t stream) {
   if (n_tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_relayout, dim3(n_tiles), dim3(256), 0, stream, home, office, tiles, to_office ? 1u : 0u);
-  return hipGetLastError();
-}
-
-hipError_t launch_export(const DevView& v, uint32_t n_sims, SimState* states, Counts7* rows, hipStream_t stream) {
-  if (n_sims == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_export, dim3(n_sims), dim3(256), 0, stream, v, states, rows);
   return hipGetLastError();
 }
 

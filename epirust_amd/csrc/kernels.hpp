@@ -30,6 +30,10 @@ struct DevView {
   const uint32_t* vacc_hour;
   const unsigned long long* vacc_thr;
   uint32_t* block_counts;  // 6 per agent block: the hour's partial S E I H R D
+  // Host-visible (pinned) copies that every hour's epilogue writes: its row (indexed like
+  // rows) and the simulation's state after it. A step call reads them after one drain.
+  Counts7* host_rows;
+  SimState* host_state;
   uint32_t office;         // 1: the agent arrays above are in the office layout
 };
 
@@ -72,13 +76,8 @@ hipError_t launch_night(const DevView& v, const LaunchShape& s, hipStream_t stre
 hipError_t launch_apply_sweep(const DevView& v, const LaunchShape& s, hipStream_t stream);
 inline bool hour_has_step(uint32_t hour) { return hour % 24u == 0 || hour % 24u >= 7; }
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t value, hipStream_t stream);
-// The host's read-back after a step call, written by one kernel into host-visible memory
-// (device addresses of pinned host buffers): every simulation's SimState to `states`, and
-// its rows not yet exported (SimState::rows_exported .. rows_written) to `rows` at the
-// same offsets as in the device's row array. A kernel's writes reach host memory sooner
-// than a small asynchronous copy each.
-hipError_t launch_export(const DevView& v, uint32_t n_sims, SimState* states, Counts7* rows, hipStream_t stream);
-// Every simulation's active flag to `active` (device address of pinned host memory).
+// Every simulation's active flag to `active` (device address of pinned host memory), for
+// the host's daily poll inside a multi-day step call.
 hipError_t launch_export_active(const DevView& v, uint32_t n_sims, uint32_t* active, hipStream_t stream);
 
 }  // namespace epi

@@ -152,8 +152,7 @@ struct SimState {
   // launch (hours of day 1..6 together) applies it while its epilogues refill the slots.
   uint32_t night_sweep, night_vacc_hour;
   uint64_t night_vacc_thr;
-  uint32_t rows_exported;   // rows already copied to the host's staging (k_export)
-  uint32_t pad;
+  uint32_t pad[2];
 };
 
 // Launch-table entry: (simulation, first hospital cell).
