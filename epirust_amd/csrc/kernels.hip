@@ -5,12 +5,15 @@
 //   [hour-of-day 0 only] k_hospital_scan : first vacant hospital cell of the snapshot
 //   k_step   : [hours of day 0, 7..23] per agent, the previous hour's vaccination /
 //              lockdown sweep, then routine + disease update against the start-of-hour
-//              grid; state written in place (it is read by no other agent), the chosen
-//              cell claimed with a 32-bit atomicMin keyed (claim hour, agent id);
-//              S/E/I/H/R/D counted with wavefront ballots into per-block partials.
+//              grid; state written in place (it is read by no other agent) when it
+//              changes, the chosen cell written to the target word and claimed with a
+//              32-bit atomicMin keyed (claim hour, agent id) unless the office filter or
+//              housemate arbitration settles it; S/E/I/H/R/D counted with wavefront
+//              ballots into per-block partials.
 //   k_commit : every hour. Its first blocks, one per simulation, run the hour's epilogue:
 //              reduce the partials, write the row, run the intervention triggers and the
-//              stop rule on device. The other blocks run the hour's agent pass, which
+//              stop rule on device; the row and the state go to device memory and to the
+//              host's pinned copies, which a step call reads after its one drain. The other blocks run the hour's agent pass, which
 //              reads nothing the same hour's epilogue writes (its hour and its sweep come
 //              from parity slots the previous epilogue filled), so the two share a launch.
 //              At hours of day 0 and 7..23 the agent pass commits the moves (the claim
@@ -33,7 +36,8 @@
 // agents.
 // Every kernel is batched: blocks map to (simulation, agent range) through a table, so
 // many independent replicates share one launch. Nothing here uses MFMA: the step is
-// integer work bound by HBM traffic.
+// integer work, priced against HBM bandwidth (its per-agent bytes are few; at 1M agents
+// the grid stays in the caches and the step is bound by VALU issue and memory latency).
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
